@@ -1,0 +1,20 @@
+"""MI355X-native BlochHighOrder free precession and multi-coil reception.
+
+The directory name is not a Python identifier, so callers load it with
+``importlib`` under the module name ``hom_amd`` (``load_package`` in
+``__graft_entry__.py``, tests/conftest.py and bench.py do exactly that).
+
+Layers:
+- ``types``      host mirror of the reference's data types and flags;
+- ``workloads``  seeded synthetic inputs (reference test workload, configs);
+- ``runtime``    ctypes binding of the C ABI in ``libhom.so`` (include/hom_abi.h);
+- ``simulate``   the reference's ``simulate`` / ``prepare_simulation`` surface
+                 on top of the ABI;
+- ``serving``    continuous-batching request loop over the ABI's server.
+"""
+from .types import (GAMMA, BlochHighOrder, DimensionMismatch, HO_DiscreteSequence,  # noqa: F401
+                    HO_Phantom, get_sim_ranges)
+from . import workloads  # noqa: F401
+
+__all__ = ["GAMMA", "BlochHighOrder", "DimensionMismatch", "HO_DiscreteSequence", "HO_Phantom",
+           "get_sim_ranges", "workloads"]
