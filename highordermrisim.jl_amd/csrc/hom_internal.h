@@ -1,0 +1,105 @@
+// hom_internal.h -- plan object and kernel launcher interfaces shared by the
+// C ABI (hom_abi.cpp) and the HIP kernel files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/hom_abi.h"
+
+namespace hom {
+
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+// Device arrays of one precision (float for HOM_F32, double otherwise).
+struct DeviceArrays {
+    void *x = nullptr, *y = nullptr, *z = nullptr, *rho = nullptr, *T1 = nullptr,
+         *T2 = nullptr, *dw = nullptr;
+    void *csm = nullptr;      // Ns x Nc complex, column stride Ns
+    void *coef = nullptr;     // 16 x n_nodes, row j contiguous
+    void *gx = nullptr, *gy = nullptr, *gz = nullptr, *dt = nullptr, *b1 = nullptr,
+         *df = nullptr, *psi = nullptr;
+    uint8_t *adc = nullptr;
+    void *mxy = nullptr, *mz = nullptr;   // state: Ns complex, Ns real
+    void *states = nullptr;               // Ns x max_adc complex (column stride Ns)
+    void *partials = nullptr;             // coil reduction partials
+    int64_t partial_capacity = 0;         // in complex elements
+};
+
+struct MixedWorkspace;  // mixed_kernels.hip
+
+struct TimerSlot {
+    std::vector<hipEvent_t> starts, stops;
+    size_t used = 0;
+    double folded_ms = 0.0;
+    int64_t folded_launches = 0;
+};
+
+}  // namespace hom
+
+struct hom_plan {
+    int device = 0;
+    int precision = HOM_F64;
+    int coil_mode = HOM_COIL_GEMM;
+    int64_t ns = 0, nc = 0, nn = 0, max_adc = 0, max_rf_adc = 0;
+    double gamma = 0.0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    hom::DeviceArrays d;
+    bool has_phantom = false, has_csm = false, has_sequence = false;
+
+    // Host copies used by the mixed path to build its integrated tables.
+    std::vector<double> h_t, h_dt, h_gx, h_gy, h_gz, h_coef;  // coef: 16 x nn
+    std::vector<uint8_t> h_adc, h_exc;
+    std::vector<double> h_csm_colmax;  // max |csm| per coil
+
+    void *signal = nullptr;
+    int64_t signal_rows = 0, signal_capacity_rows = 0;
+
+    hom::MixedWorkspace *mixed = nullptr;
+
+    bool timing = false;
+    hom::TimerSlot timers[HOM_K_COUNT];
+
+    std::vector<void *> user_buffers;
+
+    bool is_f32() const { return precision == HOM_F32; }
+    size_t real_size() const { return is_f32() ? 4 : 8; }
+    size_t complex_size() const { return 2 * real_size(); }
+};
+
+namespace hom {
+
+// Records a start/stop event pair around one launch when timing is on.
+struct ScopedTimer {
+    hom_plan *p;
+    int id;
+    hipEvent_t stop_ev = nullptr;
+    ScopedTimer(hom_plan *plan, int kernel_id);
+    ~ScopedTimer();
+};
+
+int set_error(hom_plan *p, int code, const std::string &msg);
+int check_hip(hom_plan *p, hipError_t e, const char *what);
+
+// ---- plain kernels (plain_kernels.hip) ----
+// Block [first, last] of free precession / excitation for all spins, writing
+// ADC states into d.states (column stride Ns).  Returns hipError_t.
+hipError_t launch_plain_precession(hom_plan *p, int64_t first, int64_t last, int flags);
+hipError_t launch_plain_excitation(hom_plan *p, int64_t first, int64_t last, int flags);
+// sig[k, c] = sum_s states[s, k] * csm[s, c] into out (column stride out_ld).
+hipError_t launch_coil_reduce(hom_plan *p, const void *states, int64_t ld_states, int64_t n_adc,
+                              void *out, int64_t out_ld);
+int64_t coil_partial_elements(int64_t ns, int64_t n_adc, int64_t nc);
+hipError_t launch_fill_complex(hom_plan *p, void *out, int64_t rows, int64_t cols, int64_t ld);
+hipError_t launch_reset_state(hom_plan *p);
+
+// ---- mixed path (mixed_kernels.hip) ----
+int mixed_run(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *rf, int64_t np,
+              int flags);
+void mixed_free(hom_plan *p);
+
+}  // namespace hom

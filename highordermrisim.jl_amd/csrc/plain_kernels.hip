@@ -1,0 +1,390 @@
+// plain_kernels.hip -- per-block drop-in kernels in the plan's precision
+// (f32 or f64): the reference GPU algorithm restated for CDNA4.
+//
+//   k_plain_precession   src/simulation/kernels/HighOrderGPU.jl:63-122
+//   k_plain_excitation   src/simulation/kernels/HighOrderGPU.jl:181-221
+//   k_coil_partials/     src/simulation/kernels/CoilReductionGPU.jl:16-100
+//   k_coil_finalize      (deterministic two-stage coil reduction, no atomics)
+//
+// One lane per spin, node loop in registers; node data is wave-uniform and is
+// read through the scalar cache.  The field-order mask is a template
+// parameter so each mask compiles to straight-line code without the disabled
+// terms, as the reference does with Val(ORDERS).
+#include "hom_internal.h"
+
+#include <algorithm>
+
+namespace hom {
+namespace {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) Cx {
+    T re, im;
+};
+
+template <typename T>
+__device__ inline Cx<T> cmul(Cx<T> a, Cx<T> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ inline Cx<T> cscale(Cx<T> a, T s) { return {a.re * s, a.im * s}; }
+template <typename T>
+__device__ inline Cx<T> cconj(Cx<T> a) { return {a.re, -a.im}; }
+
+__device__ inline void dsincos(double a, double *s, double *c) { sincos(a, s, c); }
+__device__ inline void dsincos(float a, float *s, float *c) { sincosf(a, s, c); }
+__device__ inline double dexp(double a) { return exp(a); }
+__device__ inline float dexp(float a) { return expf(a); }
+__device__ inline double dsqrt(double a) { return sqrt(a); }
+__device__ inline float dsqrt(float a) { return sqrtf(a); }
+template <typename T> __device__ inline T eps_of();
+template <> __device__ inline double eps_of<double>() { return 2.220446049250313e-16; }
+template <> __device__ inline float eps_of<float>() { return 1.1920929e-7f; }
+
+template <typename T>
+__device__ inline Cx<T> cis(T a) {
+    T s, c;
+    dsincos(a, &s, &c);
+    return {c, s};
+}
+
+// Masked field of one spin at one node (HighOrderGPU.jl:73-93).
+//   MASK bit 0..3: ho0..ho3, bit 5: dw_precession.
+template <typename T, int MASK>
+__device__ inline T field_at(const T (&b)[16], const T *__restrict__ coef, int64_t nn,
+                             const T *__restrict__ gx, const T *__restrict__ gy,
+                             const T *__restrict__ gz, int64_t n, T dw_term) {
+    const T bz0 = (MASK & HOM_HO0) ? coef[n] : T(0);
+    T bz1;
+    if (MASK & HOM_HO1)
+        bz1 = b[1] * coef[nn + n] + b[2] * coef[2 * nn + n] + b[3] * coef[3 * nn + n];
+    else
+        bz1 = b[1] * gx[n] + b[2] * gy[n] + b[3] * gz[n];
+    T bz2 = T(0), bz3 = T(0);
+    if (MASK & HOM_HO2)
+        bz2 = b[4] * coef[4 * nn + n] + b[5] * coef[5 * nn + n] + b[6] * coef[6 * nn + n] +
+              b[7] * coef[7 * nn + n] + b[8] * coef[8 * nn + n];
+    if (MASK & HOM_HO3)
+        bz3 = b[9] * coef[9 * nn + n] + b[10] * coef[10 * nn + n] + b[11] * coef[11 * nn + n] +
+              b[12] * coef[12 * nn + n] + b[13] * coef[13 * nn + n] + b[14] * coef[14 * nn + n] +
+              b[15] * coef[15 * nn + n];
+    const T bz = bz0 + bz1 + bz2 + bz3;
+    return (MASK & HOM_DW_PRECESSION) ? bz + dw_term : bz;
+}
+
+// Unnormalised real solid harmonics through third order (HighOrderGPU.jl:63-71).
+template <typename T>
+__device__ inline void basis16(T x, T y, T z, T (&b)[16]) {
+    const T x2 = x * x, y2 = y * y, z2 = z * z;
+    const T r2 = x2 + y2 + z2;
+    b[0] = T(1); b[1] = x; b[2] = y; b[3] = z;
+    b[4] = x * y; b[5] = z * y; b[6] = T(3) * z2 - r2; b[7] = x * z; b[8] = x2 - y2;
+    b[9] = T(3) * y * x2 - y * y2; b[10] = x * z * y; b[11] = (T(5) * z2 - r2) * y;
+    b[12] = T(5) * z * z2 - T(3) * z * r2; b[13] = (T(5) * z2 - r2) * x;
+    b[14] = x2 * z - y2 * z; b[15] = x * x2 - T(3) * x * y2;
+}
+
+template <typename T, int MASK>
+__global__ __launch_bounds__(256) void k_plain_precession(
+    int64_t ns, const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
+    const T *__restrict__ rho, const T *__restrict__ T1, const T *__restrict__ T2,
+    const T *__restrict__ dw, const T *__restrict__ coef, int64_t nn, const T *__restrict__ gx,
+    const T *__restrict__ gy, const T *__restrict__ gz, const T *__restrict__ dt,
+    const uint8_t *__restrict__ adc, int64_t first, int64_t last, T kphase, T inv2pig,
+    Cx<T> *__restrict__ mxy, T *__restrict__ mz, Cx<T> *__restrict__ states, int64_t ld) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    T b[16];
+    basis16(x[s], y[s], z[s], b);
+    const T dw_term = dw[s] * inv2pig;
+    const T t2 = T2[s];
+    T bprev = field_at<T, MASK>(b, coef, nn, gx, gy, gz, first, dw_term);
+    const Cx<T> initial = mxy[s];
+    T phi = T(0), t = T(0);
+    int64_t sample = 0;
+    for (int64_t n = first + 1; n <= last; ++n) {
+        const T d = dt[n - 1];
+        t += d;
+        const T bnext = field_at<T, MASK>(b, coef, nn, gx, gy, gz, n, dw_term);
+        phi += kphase * (bprev + bnext) * d;
+        if (adc[n]) {
+            states[s + sample * ld] = cmul(cscale(initial, dexp(-t / t2)), cis(phi));
+            ++sample;
+        }
+        bprev = bnext;
+    }
+    mxy[s] = cmul(cscale(initial, dexp(-t / t2)), cis(phi));
+    const T e1 = dexp(-t / T1[s]);
+    mz[s] = mz[s] * e1 + rho[s] * (T(1) - e1);
+}
+
+// Hard-pulse spinor update (HighOrderGPU.jl:181-192).
+template <typename T>
+__device__ inline void rotate_spin(Cx<T> &m, T &mzv, Cx<T> b1, T bz, T dt, T k2pig) {
+    T B = dsqrt(b1.re * b1.re + b1.im * b1.im + bz * bz);
+    B = (B == T(0)) ? eps_of<T>() : B;
+    const T phi = k2pig * (B * dt);
+    T s, c;
+    dsincos(phi / T(2), &s, &c);
+    const Cx<T> a = {c, -(bz / B) * s};
+    const Cx<T> nb = {b1.re / B * s, b1.im / B * s};
+    const Cx<T> beta = {nb.im, -nb.re};  // -i * (B1/B) * s
+    const Cx<T> ca = cconj(a);
+    const Cx<T> t1 = cscale(cmul(ca, beta), T(2) * mzv);
+    const Cx<T> t2 = cmul(cmul(ca, ca), m);
+    const Cx<T> t3 = cmul(cmul(beta, beta), cconj(m));
+    const Cx<T> ab = cmul(cmul(a, beta), cconj(m));
+    const T na = a.re * a.re + a.im * a.im, nbeta = beta.re * beta.re + beta.im * beta.im;
+    m = {t1.re + t2.re - t3.re, t1.im + t2.im - t3.im};
+    mzv = (na - nbeta) * mzv - T(2) * ab.re;
+}
+
+template <typename T, bool DWX>
+__global__ __launch_bounds__(256) void k_plain_excitation(
+    int64_t ns, const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
+    const T *__restrict__ rho, const T *__restrict__ T1, const T *__restrict__ T2,
+    const T *__restrict__ dw, const T *__restrict__ gx, const T *__restrict__ gy,
+    const T *__restrict__ gz, const Cx<T> *__restrict__ b1, const T *__restrict__ df,
+    const T *__restrict__ psi, const T *__restrict__ dt, const uint8_t *__restrict__ adc,
+    int64_t first, int64_t last, T k2pig, T inv2pig, T invg, Cx<T> *__restrict__ mxy,
+    T *__restrict__ mz, Cx<T> *__restrict__ states, int64_t ld) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    Cx<T> m = cmul(mxy[s], cis(-psi[first]));
+    T mzv = mz[s];
+    const T off = DWX ? dw[s] * inv2pig : T(0);
+    const T xs = x[s], ys = y[s], zs = z[s], t1s = T1[s], t2s = T2[s], rs = rho[s];
+    int64_t sample = 0;
+    for (int64_t n = first; n < last; ++n) {
+        const T d = dt[n];
+        const T bz = gx[n] * xs + gy[n] * ys + gz[n] * zs + off - df[n] * invg;
+        rotate_spin(m, mzv, b1[n], bz, d, k2pig);
+        m = cscale(m, dexp(-d / t2s));
+        const T e1 = dexp(-d / t1s);
+        mzv = mzv * e1 + rs * (T(1) - e1);
+        if (adc[n + 1]) {
+            states[s + sample * ld] = cmul(m, cis(psi[n + 1]));
+            ++sample;
+        }
+    }
+    mxy[s] = cmul(m, cis(psi[last]));
+    mz[s] = mzv;
+}
+
+// Deterministic coil reduction, stage 1: one workgroup per (spin tile of
+// kSpinTile spins, 32-sample x 32-coil channel tile).  Each lane owns one
+// sample and four coils of the tile; spins stream through LDS 32 at a time.
+constexpr int kSpinTile = 2048;
+constexpr int kChunk = 32;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_coil_partials(
+    int64_t ns, int64_t nadc, int64_t nc, const Cx<T> *__restrict__ states, int64_t lds,
+    const Cx<T> *__restrict__ csm, int64_t ldc, Cx<T> *__restrict__ partials, int64_t nch) {
+    __shared__ Cx<T> st[kChunk][kChunk + 1];
+    __shared__ Cx<T> cs[kChunk][kChunk + 1];
+    const int tid = threadIdx.x;
+    const int64_t tile = blockIdx.x;
+    const int64_t ktiles = (nadc + 31) / 32;
+    const int64_t k0 = (blockIdx.y % ktiles) * 32, c0 = (blockIdx.y / ktiles) * 32;
+    const int kk = tid % 32, cq = tid / 32;  // sample kk, coils cq + 8j
+    Cx<T> acc[4] = {};
+    const int64_t s_begin = tile * kSpinTile;
+    const int64_t s_end = std::min<int64_t>(ns, s_begin + kSpinTile);
+    for (int64_t sc = s_begin; sc < s_end; sc += kChunk) {
+        for (int j = 0; j < 4; ++j) {
+            const int row = tid % 32, col = tid / 32 + 8 * j;
+            const int64_t s = sc + row;
+            const int64_t k = k0 + col, c = c0 + col;
+            st[row][col] = (s < s_end && k < nadc) ? states[s + k * lds] : Cx<T>{};
+            cs[row][col] = (s < s_end && c < nc) ? csm[s + c * ldc] : Cx<T>{};
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int r = 0; r < kChunk; ++r) {
+            const Cx<T> a = st[r][kk];
+            for (int j = 0; j < 4; ++j) {
+                const Cx<T> w = cs[r][cq + 8 * j];
+                acc[j].re += a.re * w.re - a.im * w.im;
+                acc[j].im += a.re * w.im + a.im * w.re;
+            }
+        }
+        __syncthreads();
+    }
+    for (int j = 0; j < 4; ++j) {
+        const int64_t k = k0 + kk, c = c0 + cq + 8 * j;
+        if (k < nadc && c < nc) partials[tile * nch + (k + c * nadc)] = acc[j];
+    }
+}
+
+// Stage 2: each lane sums one (sample, coil) channel over the spin tiles in
+// tile order and writes it to the (possibly strided) output.
+template <typename T>
+__global__ __launch_bounds__(256) void k_coil_finalize(int64_t ntiles, int64_t nadc, int64_t nch,
+                                                      const Cx<T> *__restrict__ partials,
+                                                      Cx<T> *__restrict__ out, int64_t ldo) {
+    const int64_t ch = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ch >= nch) return;
+    Cx<T> v = {};
+    for (int64_t t = 0; t < ntiles; ++t) {
+        const Cx<T> p = partials[t * nch + ch];
+        v.re += p.re;
+        v.im += p.im;
+    }
+    const int64_t k = ch % nadc, c = ch / nadc;
+    out[k + c * ldo] = v;
+}
+
+template <typename T>
+__global__ void k_fill_zero(Cx<T> *out, int64_t rows, int64_t cols, int64_t ld) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows * cols) return;
+    out[(i % rows) + (i / rows) * ld] = Cx<T>{};
+}
+
+template <typename T>
+__global__ void k_reset_state(int64_t ns, const T *rho, Cx<T> *mxy, T *mz) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    mxy[i] = Cx<T>{};
+    mz[i] = rho[i];
+}
+
+inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+template <typename T>
+hipError_t precession_t(hom_plan *p, int64_t first, int64_t last, int flags) {
+    const int mask = flags & (HOM_HO0 | HOM_HO1 | HOM_HO2 | HOM_HO3 | HOM_DW_PRECESSION);
+    const auto &d = p->d;
+    const T kphase = T(-M_PI * p->gamma);
+    const T inv2pig = T(1) / T(kTwoPi * p->gamma);
+    if (p->ns == 0) return hipSuccess;
+    dim3 grid(grid_for(p->ns, 256)), block(256);
+#define HOM_CASE(M)                                                                           \
+    case M:                                                                                   \
+        hipLaunchKernelGGL((k_plain_precession<T, M>), grid, block, 0, p->stream, p->ns,      \
+                           (const T *)d.x, (const T *)d.y, (const T *)d.z, (const T *)d.rho,  \
+                           (const T *)d.T1, (const T *)d.T2, (const T *)d.dw,                 \
+                           (const T *)d.coef, p->nn, (const T *)d.gx, (const T *)d.gy,        \
+                           (const T *)d.gz, (const T *)d.dt, d.adc, first, last, kphase,      \
+                           inv2pig, (Cx<T> *)d.mxy, (T *)d.mz, (Cx<T> *)d.states, p->ns);     \
+        break;
+    ScopedTimer timer(p, HOM_K_PLAIN_PRECESSION);
+    switch (mask) {
+        HOM_CASE(0) HOM_CASE(1) HOM_CASE(2) HOM_CASE(3) HOM_CASE(4) HOM_CASE(5) HOM_CASE(6)
+        HOM_CASE(7) HOM_CASE(8) HOM_CASE(9) HOM_CASE(10) HOM_CASE(11) HOM_CASE(12)
+        HOM_CASE(13) HOM_CASE(14) HOM_CASE(15) HOM_CASE(32) HOM_CASE(33) HOM_CASE(34)
+        HOM_CASE(35) HOM_CASE(36) HOM_CASE(37) HOM_CASE(38) HOM_CASE(39) HOM_CASE(40)
+        HOM_CASE(41) HOM_CASE(42) HOM_CASE(43) HOM_CASE(44) HOM_CASE(45) HOM_CASE(46)
+        HOM_CASE(47)
+        default: return hipErrorInvalidValue;
+    }
+#undef HOM_CASE
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t excitation_t(hom_plan *p, int64_t first, int64_t last, int flags) {
+    const auto &d = p->d;
+    if (p->ns == 0) return hipSuccess;
+    const T k2pig = T(-kTwoPi * p->gamma);
+    const T inv2pig = T(1) / T(kTwoPi * p->gamma);
+    const T invg = T(1) / T(p->gamma);
+    dim3 grid(grid_for(p->ns, 256)), block(256);
+    ScopedTimer timer(p, HOM_K_PLAIN_EXCITATION);
+    if (flags & HOM_DW_EXCITATION)
+        hipLaunchKernelGGL((k_plain_excitation<T, true>), grid, block, 0, p->stream, p->ns,
+                           (const T *)d.x, (const T *)d.y, (const T *)d.z, (const T *)d.rho,
+                           (const T *)d.T1, (const T *)d.T2, (const T *)d.dw, (const T *)d.gx,
+                           (const T *)d.gy, (const T *)d.gz, (const Cx<T> *)d.b1,
+                           (const T *)d.df, (const T *)d.psi, (const T *)d.dt, d.adc, first, last,
+                           k2pig, inv2pig, invg, (Cx<T> *)d.mxy, (T *)d.mz, (Cx<T> *)d.states,
+                           p->ns);
+    else
+        hipLaunchKernelGGL((k_plain_excitation<T, false>), grid, block, 0, p->stream, p->ns,
+                           (const T *)d.x, (const T *)d.y, (const T *)d.z, (const T *)d.rho,
+                           (const T *)d.T1, (const T *)d.T2, (const T *)d.dw, (const T *)d.gx,
+                           (const T *)d.gy, (const T *)d.gz, (const Cx<T> *)d.b1,
+                           (const T *)d.df, (const T *)d.psi, (const T *)d.dt, d.adc, first, last,
+                           k2pig, inv2pig, invg, (Cx<T> *)d.mxy, (T *)d.mz, (Cx<T> *)d.states,
+                           p->ns);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t reduce_t(hom_plan *p, const void *states, int64_t lds, int64_t nadc, void *out,
+                    int64_t ldo) {
+    const int64_t nc = p->nc;
+    if (nadc == 0 || nc == 0) return hipSuccess;
+    if (p->ns == 0) {
+        hipLaunchKernelGGL((k_fill_zero<T>), dim3(grid_for(nadc * nc, 256)), dim3(256), 0,
+                           p->stream, (Cx<T> *)out, nadc, nc, ldo);
+        return hipGetLastError();
+    }
+    const int64_t ntiles = (p->ns + kSpinTile - 1) / kSpinTile;
+    const int64_t nch = nadc * nc;
+    const int64_t ktiles = (nadc + 31) / 32, ctiles = (nc + 31) / 32;
+    {
+        ScopedTimer timer(p, HOM_K_COIL_PARTIALS);
+        hipLaunchKernelGGL((k_coil_partials<T>), dim3((unsigned)ntiles, (unsigned)(ktiles * ctiles)),
+                           dim3(256), 0, p->stream, p->ns, nadc, nc, (const Cx<T> *)states, lds,
+                           (const Cx<T> *)p->d.csm, p->ns, (Cx<T> *)p->d.partials, nch);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    {
+        ScopedTimer timer(p, HOM_K_COIL_FINALIZE);
+        hipLaunchKernelGGL((k_coil_finalize<T>), dim3(grid_for(nch, 256)), dim3(256), 0, p->stream,
+                           ntiles, nadc, nch, (const Cx<T> *)p->d.partials, (Cx<T> *)out, ldo);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+int64_t coil_partial_elements(int64_t ns, int64_t n_adc, int64_t nc) {
+    const int64_t ntiles = (ns + kSpinTile - 1) / kSpinTile;
+    return std::max<int64_t>(ntiles, 1) * std::max<int64_t>(n_adc, 1) * std::max<int64_t>(nc, 1);
+}
+
+hipError_t launch_plain_precession(hom_plan *p, int64_t first, int64_t last, int flags) {
+    return p->is_f32() ? precession_t<float>(p, first, last, flags)
+                       : precession_t<double>(p, first, last, flags);
+}
+
+hipError_t launch_plain_excitation(hom_plan *p, int64_t first, int64_t last, int flags) {
+    return p->is_f32() ? excitation_t<float>(p, first, last, flags)
+                       : excitation_t<double>(p, first, last, flags);
+}
+
+hipError_t launch_coil_reduce(hom_plan *p, const void *states, int64_t ld_states, int64_t n_adc,
+                              void *out, int64_t out_ld) {
+    return p->is_f32() ? reduce_t<float>(p, states, ld_states, n_adc, out, out_ld)
+                       : reduce_t<double>(p, states, ld_states, n_adc, out, out_ld);
+}
+
+hipError_t launch_fill_complex(hom_plan *p, void *out, int64_t rows, int64_t cols, int64_t ld) {
+    if (rows * cols == 0) return hipSuccess;
+    if (p->is_f32())
+        hipLaunchKernelGGL((k_fill_zero<float>), dim3(grid_for(rows * cols, 256)), dim3(256), 0,
+                           p->stream, (Cx<float> *)out, rows, cols, ld);
+    else
+        hipLaunchKernelGGL((k_fill_zero<double>), dim3(grid_for(rows * cols, 256)), dim3(256), 0,
+                           p->stream, (Cx<double> *)out, rows, cols, ld);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset_state(hom_plan *p) {
+    if (p->ns == 0) return hipSuccess;
+    if (p->is_f32())
+        hipLaunchKernelGGL((k_reset_state<float>), dim3(grid_for(p->ns, 256)), dim3(256), 0,
+                           p->stream, p->ns, (const float *)p->d.rho, (Cx<float> *)p->d.mxy,
+                           (float *)p->d.mz);
+    else
+        hipLaunchKernelGGL((k_reset_state<double>), dim3(grid_for(p->ns, 256)), dim3(256), 0,
+                           p->stream, p->ns, (const double *)p->d.rho, (Cx<double> *)p->d.mxy,
+                           (double *)p->d.mz);
+    return hipGetLastError();
+}
+
+}  // namespace hom

@@ -1,0 +1,303 @@
+"""ctypes binding of libhom.so (the C ABI in include/hom_abi.h).
+
+This is the Python counterpart of the Julia ``ccall`` shim: host arrays go in
+as plain pointers with Julia's layouts (column-major, interleaved complex), and
+status codes come back as exceptions -- ``ValueError`` for HOM_ERR_ARGUMENT
+(Julia ArgumentError), ``DimensionMismatch`` for HOM_ERR_DIMENSION and
+``HomDeviceError`` for the rest.
+
+The library must have been built (``build.py``); loading fails loudly when it
+is missing, and nothing here falls back to a CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from .types import GAMMA, BlochHighOrder, DimensionMismatch, HO_DiscreteSequence, HO_Phantom
+
+LIB_PATH = Path(__file__).resolve().parent / "libhom.so"
+
+HOM_OK, HOM_ERR_ARGUMENT, HOM_ERR_DIMENSION, HOM_ERR_DEVICE, HOM_ERR_MEMORY, HOM_ERR_STATE = range(6)
+PRECISIONS = {"f32": 0, "f64": 1, "mixed": 2}
+COIL_MODES = {"gemm": 0, "reduction": 1, "fused_warp": 2, "fused_tiled": 3}
+KERNELS = {"plain_precession": 0, "plain_excitation": 1, "coil_partials": 2, "coil_finalize": 3,
+           "mixed_segment": 4, "mixed_finalize": 5, "mixed_prep": 6}
+
+# Every symbol declared in include/hom_abi.h.
+ABI_SYMBOLS = (
+    "hom_abi_version", "hom_device_count", "hom_last_error", "hom_plan_create", "hom_plan_destroy",
+    "hom_upload_phantom", "hom_upload_csm", "hom_upload_sequence", "hom_reset_state",
+    "hom_set_state", "hom_get_state", "hom_precession_block", "hom_excitation_block",
+    "hom_reduce_signal", "hom_run", "hom_synchronize", "hom_signal_rows", "hom_download_signal",
+    "hom_signal_element_size", "hom_buffer_alloc", "hom_buffer_free", "hom_buffer_upload",
+    "hom_buffer_download", "hom_plan_set_timing", "hom_plan_kernel_time",
+)
+
+
+class HomDeviceError(RuntimeError):
+    """Device, memory or state failure reported by the library."""
+
+
+_dp = C.POINTER(C.c_double)
+_u8p = C.POINTER(C.c_uint8)
+_i64p = C.POINTER(C.c_int64)
+_vp = C.c_void_p
+
+
+class SequenceHost(C.Structure):
+    _fields_ = [("n_nodes", C.c_int64), ("t", _dp), ("dt", _dp), ("gx", _dp), ("gy", _dp),
+                ("gz", _dp), ("b1", _dp), ("df", _dp), ("psi", _dp), ("adc", _u8p),
+                ("excitation", _u8p), ("h", _dp * 16)]
+
+
+_lib = None
+
+
+def load_library():
+    """Loads libhom.so and declares every ABI signature."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise HomDeviceError(f"{LIB_PATH} is missing: build it with build.py "
+                             "(the HIP extension is required; there is no CPU fallback)")
+    lib = C.CDLL(str(LIB_PATH))
+    P = _vp
+    sig = {
+        "hom_abi_version": (C.c_int, []),
+        "hom_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+        "hom_last_error": (C.c_char_p, [P]),
+        "hom_plan_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.c_int64, C.c_int64,
+                                      C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_int]),
+        "hom_plan_destroy": (C.c_int, [P]),
+        "hom_upload_phantom": (C.c_int, [P, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "hom_upload_csm": (C.c_int, [P, _dp, C.c_int64]),
+        "hom_upload_sequence": (C.c_int, [P, C.POINTER(SequenceHost)]),
+        "hom_reset_state": (C.c_int, [P]),
+        "hom_set_state": (C.c_int, [P, _dp, _dp]),
+        "hom_get_state": (C.c_int, [P, _dp, _dp]),
+        "hom_precession_block": (C.c_int, [P, C.c_int64, C.c_int64, C.c_int, _vp, C.c_int64]),
+        "hom_excitation_block": (C.c_int, [P, C.c_int64, C.c_int64, C.c_int, _vp, C.c_int64]),
+        "hom_reduce_signal": (C.c_int, [P, _vp, C.c_int64, C.c_int64, _vp, C.c_int64]),
+        "hom_run": (C.c_int, [P, _i64p, _i64p, _u8p, C.c_int64, C.c_int]),
+        "hom_synchronize": (C.c_int, [P]),
+        "hom_signal_rows": (C.c_int, [P, _i64p]),
+        "hom_download_signal": (C.c_int, [P, _vp, C.c_int64]),
+        "hom_signal_element_size": (C.c_int, [P, _i64p]),
+        "hom_buffer_alloc": (C.c_int, [P, C.c_int64, C.POINTER(_vp)]),
+        "hom_buffer_free": (C.c_int, [P, _vp]),
+        "hom_buffer_upload": (C.c_int, [P, _vp, _vp, C.c_int64]),
+        "hom_buffer_download": (C.c_int, [P, _vp, _vp, C.c_int64]),
+        "hom_plan_set_timing": (C.c_int, [P, C.c_int]),
+        "hom_plan_kernel_time": (C.c_int, [P, C.c_int, _dp, _i64p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def device_count() -> int:
+    lib = load_library()
+    n = C.c_int(0)
+    lib.hom_device_count(C.byref(n))
+    return n.value
+
+
+def _raise(code: int, msg: str):
+    if code == HOM_ERR_ARGUMENT:
+        raise ValueError(msg)
+    if code == HOM_ERR_DIMENSION:
+        raise DimensionMismatch(msg)
+    raise HomDeviceError(msg)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class DeviceBuffer:
+    """Plan-owned device memory (for hosts without a GPU array library)."""
+
+    def __init__(self, plan: "Plan", nbytes: int):
+        self.plan = plan
+        self.nbytes = int(nbytes)
+        p = _vp()
+        plan._check(plan.lib.hom_buffer_alloc(plan.handle, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, host: np.ndarray):
+        host = np.ascontiguousarray(host) if host.flags.c_contiguous or not host.flags.f_contiguous \
+            else host
+        if host.nbytes > self.nbytes:
+            raise DimensionMismatch("host array larger than the device buffer")
+        self.plan._check(self.plan.lib.hom_buffer_upload(self.plan.handle, self.ptr,
+                                                         host.ctypes.data, host.nbytes))
+
+    def download(self, host: np.ndarray):
+        if host.nbytes > self.nbytes:
+            raise DimensionMismatch("host array larger than the device buffer")
+        self.plan._check(self.plan.lib.hom_buffer_download(self.plan.handle, host.ctypes.data,
+                                                           self.ptr, host.nbytes))
+        return host
+
+    def free(self):
+        if self.ptr:
+            self.plan._check(self.plan.lib.hom_buffer_free(self.plan.handle, self.ptr))
+            self.ptr = None
+
+
+class Plan:
+    """One device-resident simulation plan: phantom, CSM, sequence, spin
+    state, ADC-state workspace and signal buffer on one GPU and one stream."""
+
+    def __init__(self, precision: str, n_spins: int, n_coils: int, n_nodes: int, max_adc: int,
+                 max_rf_adc: int | None = None, gamma: float = GAMMA,
+                 coil_combination: str = "gemm", device: int = 0):
+        if precision not in PRECISIONS:
+            raise ValueError("The precision must be f32, f64 or mixed.")
+        if coil_combination not in COIL_MODES:
+            raise ValueError(f"Unknown coil_combination: {coil_combination}.")
+        self.lib = load_library()
+        self.precision = precision
+        self.n_spins, self.n_coils, self.n_nodes = int(n_spins), int(n_coils), int(n_nodes)
+        self.max_adc = int(max_adc)
+        self.max_rf_adc = self.max_adc if max_rf_adc is None else int(max_rf_adc)
+        self.gamma = float(gamma)
+        h = _vp()
+        rc = self.lib.hom_plan_create(C.byref(h), int(device), PRECISIONS[precision], self.n_spins,
+                                      self.n_coils, self.n_nodes, self.max_adc, self.max_rf_adc,
+                                      self.gamma, COIL_MODES[coil_combination])
+        if rc != HOM_OK:
+            _raise(rc, self.lib.hom_last_error(None).decode())
+        self.handle = h.value
+        self.cdtype = np.complex64 if precision == "f32" else np.complex128
+        self._keep = None
+
+    # -- lifecycle ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.hom_plan_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != HOM_OK:
+            _raise(rc, self.lib.hom_last_error(self.handle).decode())
+
+    # -- residency ---------------------------------------------------------
+    def upload_phantom(self, obj: HO_Phantom):
+        obj.validate()
+        if len(obj) != self.n_spins or obj.n_coils != self.n_coils:
+            raise DimensionMismatch("The phantom does not match the plan's spin and coil counts.")
+        arrs = [_f64(a) for a in (obj.x, obj.y, obj.z, obj.rho, obj.T1, obj.T2, obj.dw)]
+        self._check(self.lib.hom_upload_phantom(self.handle, *[a.ctypes.data_as(_dp) for a in arrs]))
+        csm = np.asfortranarray(obj.csm, dtype=np.complex128)
+        self._check(self.lib.hom_upload_csm(self.handle, csm.ctypes.data_as(_dp),
+                                            max(csm.shape[0], 1)))
+
+    def upload_sequence(self, seq: HO_DiscreteSequence):
+        seq.validate()
+        if seq.n_nodes != self.n_nodes:
+            raise DimensionMismatch("The sequence node count does not match the plan.")
+        keep = [_f64(seq.t), _f64(seq.dt if seq.dt.size else np.zeros(1)), _f64(seq.gx),
+                _f64(seq.gy), _f64(seq.gz), np.ascontiguousarray(seq.b1, np.complex128),
+                _f64(seq.df), _f64(seq.psi)]
+        adc = np.ascontiguousarray(seq.adc, np.uint8)
+        exc = np.ascontiguousarray(seq.excitation if seq.excitation.size else np.zeros(1), np.uint8)
+        h = np.ascontiguousarray(seq.h, np.float64)
+        hp = (_dp * 16)(*[C.cast(h.ctypes.data + j * h.strides[0], _dp) for j in range(16)])
+        s = SequenceHost(seq.n_nodes, *[a.ctypes.data_as(_dp) for a in keep],
+                         adc.ctypes.data_as(_u8p), exc.ctypes.data_as(_u8p), hp)
+        self._check(self.lib.hom_upload_sequence(self.handle, C.byref(s)))
+
+    def reset_state(self):
+        self._check(self.lib.hom_reset_state(self.handle))
+
+    def set_state(self, mxy, mz):
+        mxy = np.ascontiguousarray(mxy, np.complex128)
+        mz = _f64(mz)
+        if mxy.size != self.n_spins or mz.size != self.n_spins:
+            raise DimensionMismatch("State arrays must contain one value per spin.")
+        self._check(self.lib.hom_set_state(self.handle, mxy.ctypes.data_as(_dp), mz.ctypes.data_as(_dp)))
+
+    def get_state(self):
+        mxy = np.zeros(self.n_spins, np.complex128)
+        mz = np.zeros(self.n_spins, np.float64)
+        self._check(self.lib.hom_get_state(self.handle, mxy.ctypes.data_as(_dp), mz.ctypes.data_as(_dp)))
+        return mxy, mz
+
+    # -- compute -------------------------------------------------------------
+    @staticmethod
+    def _flags(method) -> int:
+        return method.flags if isinstance(method, BlochHighOrder) else int(method)
+
+    def precession_block(self, first, last, method, out: DeviceBuffer | None, ld: int,
+                         offset_elems: int = 0):
+        ptr = None if out is None else out.ptr + offset_elems * np.dtype(self.cdtype).itemsize
+        self._check(self.lib.hom_precession_block(self.handle, int(first), int(last),
+                                                  self._flags(method), ptr, int(ld)))
+
+    def excitation_block(self, first, last, method, out: DeviceBuffer | None, ld: int,
+                         offset_elems: int = 0):
+        ptr = None if out is None else out.ptr + offset_elems * np.dtype(self.cdtype).itemsize
+        self._check(self.lib.hom_excitation_block(self.handle, int(first), int(last),
+                                                  self._flags(method), ptr, int(ld)))
+
+    def reduce_signal(self, states: DeviceBuffer, ld_states: int, n_adc: int,
+                      out: DeviceBuffer, ld: int, offset_elems: int = 0):
+        ptr = out.ptr + offset_elems * np.dtype(self.cdtype).itemsize
+        self._check(self.lib.hom_reduce_signal(self.handle, states.ptr if states else None,
+                                               int(ld_states), int(n_adc), ptr, int(ld)))
+
+    def run(self, parts, method):
+        pf = np.ascontiguousarray(parts[0], np.int64)
+        pl = np.ascontiguousarray(parts[1], np.int64)
+        rf = np.ascontiguousarray(parts[2], np.uint8)
+        self._keep = (pf, pl, rf)
+        self._check(self.lib.hom_run(self.handle, pf.ctypes.data_as(_i64p), pl.ctypes.data_as(_i64p),
+                                     rf.ctypes.data_as(_u8p), pf.size, self._flags(method)))
+
+    def synchronize(self):
+        self._check(self.lib.hom_synchronize(self.handle))
+
+    def signal_rows(self) -> int:
+        r = C.c_int64(0)
+        self._check(self.lib.hom_signal_rows(self.handle, C.byref(r)))
+        return r.value
+
+    def download_signal(self) -> np.ndarray:
+        rows = self.signal_rows()
+        out = np.zeros((rows, self.n_coils), dtype=self.cdtype, order="F")
+        if rows:
+            self._check(self.lib.hom_download_signal(self.handle, out.ctypes.data, rows))
+        return out
+
+    def buffer(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    # -- timing --------------------------------------------------------------
+    def set_timing(self, enable: bool):
+        self._check(self.lib.hom_plan_set_timing(self.handle, 1 if enable else 0))
+
+    def kernel_time(self, name: str):
+        ms = C.c_double(0)
+        n = C.c_int64(0)
+        self._check(self.lib.hom_plan_kernel_time(self.handle, KERNELS[name], C.byref(ms), C.byref(n)))
+        return ms.value, n.value

@@ -187,7 +187,10 @@ class HO_DiscreteSequence:
         self.df, self.psi = f64(self.df), f64(self.psi)
         self.adc = np.ascontiguousarray(np.asarray(self.adc, dtype=bool).reshape(-1))
         self.excitation = np.ascontiguousarray(np.asarray(self.excitation, dtype=bool).reshape(-1))
-        self.h = np.ascontiguousarray(np.asarray(self.h, dtype=np.float64).reshape(16, -1))
+        h = np.asarray(self.h, dtype=np.float64)
+        if h.shape != (16, n):
+            raise DimensionMismatch("Sixteen coefficient waveforms with one value per node are required.")
+        self.h = np.ascontiguousarray(h)
         self.validate(n)
 
     def validate(self, n=None) -> None:

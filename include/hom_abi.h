@@ -1,0 +1,166 @@
+/*
+ * hom_abi.h -- C ABI of the MI355X BlochHighOrder precession + reception
+ * library (libhom.so).  Plain pointers and sizes only; every call returns a
+ * status (HOM_OK = 0) and the text of the last error is available from
+ * hom_last_error().
+ *
+ * Conventions (Julia layouts, so a ccall passes its arrays unchanged):
+ *   - real per-spin and per-node arrays are double on the host;
+ *   - complex arrays are interleaved {re, im} of the signal element type;
+ *   - matrices are column-major with an explicit leading dimension;
+ *   - node indices are 0-based and a block is the inclusive node range
+ *     [first, last]; the ADC flag of the shared first node belongs to the
+ *     previous block (reference src/simulation/SimulatorCore.jl:115-117).
+ *
+ * One plan is bound to one device and one HIP stream and owns every device
+ * allocation it makes.  Calls on one plan must be serialized by the caller;
+ * different plans may be used from different threads.  Shapes are validated
+ * before any allocation or launch.
+ */
+#ifndef HOM_ABI_H
+#define HOM_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HOM_ABI_VERSION 1
+
+/* Status codes.  The Julia shim maps them to exceptions:
+ * ARGUMENT -> ArgumentError, DIMENSION -> DimensionMismatch, rest -> ErrorException. */
+enum hom_status {
+    HOM_OK = 0,
+    HOM_ERR_ARGUMENT = 1,
+    HOM_ERR_DIMENSION = 2,
+    HOM_ERR_DEVICE = 3,
+    HOM_ERR_MEMORY = 4,
+    HOM_ERR_STATE = 5
+};
+
+/* "precision" of the reference's sim_params (Simulate.jl:242-243), plus the
+ * mixed-precision fast path (FP64 phase + split-FP16 MFMA coil reduction). */
+enum hom_precision { HOM_F32 = 0, HOM_F64 = 1, HOM_MIXED = 2 };
+
+/* "coil_combination" (reference HighOrderGPU.jl:4). */
+enum hom_coil_mode {
+    HOM_COIL_GEMM = 0,
+    HOM_COIL_REDUCTION = 1,
+    HOM_COIL_FUSED_WARP = 2,
+    HOM_COIL_FUSED_TILED = 3
+};
+
+/* BlochHighOrder flags (reference src/simulation/BlochHighOrder.jl:155-162). */
+enum hom_flags {
+    HOM_HO0 = 1 << 0,
+    HOM_HO1 = 1 << 1,
+    HOM_HO2 = 1 << 2,
+    HOM_HO3 = 1 << 3,
+    HOM_DW_EXCITATION = 1 << 4,
+    HOM_DW_PRECESSION = 1 << 5
+};
+
+/* Kernel ids for hom_plan_kernel_time. */
+enum hom_kernel_id {
+    HOM_K_PLAIN_PRECESSION = 0,
+    HOM_K_PLAIN_EXCITATION = 1,
+    HOM_K_COIL_PARTIALS = 2,
+    HOM_K_COIL_FINALIZE = 3,
+    HOM_K_MIXED_SEGMENT = 4,
+    HOM_K_MIXED_FINALIZE = 5,
+    HOM_K_MIXED_PREP = 6,
+    HOM_K_COUNT = 7
+};
+
+typedef struct hom_plan hom_plan;
+
+/* Sampled sequence on the host: KomaMRI DiscreteSequence node arrays plus the
+ * 16 coefficient waveforms (reference src/datatypes/HO_DiscreteSequence.jl:31-49).
+ * t, gx, gy, gz, df, psi, adc and h[j] have n_nodes entries; b1 has n_nodes
+ * complex entries; dt and excitation have n_nodes - 1. */
+typedef struct hom_sequence_host {
+    int64_t n_nodes;
+    const double *t, *dt, *gx, *gy, *gz, *b1, *df, *psi;
+    const uint8_t *adc, *excitation;
+    const double *h[16];
+} hom_sequence_host;
+
+int hom_abi_version(void);
+int hom_device_count(int *count);
+
+/* Last error of this plan; with plan == NULL, the calling thread's last
+ * error from hom_plan_create. */
+const char *hom_last_error(const hom_plan *plan);
+
+/* Replaces the HighOrderGPUWorkspace constructor (reference
+ * src/simulation/kernels/HighOrderGPU.jl:16-58): same argument checks
+ * (0 <= max_rf_adc <= max_adc, known coil mode) plus the device, precision,
+ * problem sizes and gamma (Hz/T), which the Julia side carries in its types. */
+int hom_plan_create(hom_plan **plan, int device, int precision, int64_t n_spins, int64_t n_coils,
+                    int64_t n_nodes, int64_t max_adc, int64_t max_rf_adc, double gamma,
+                    int coil_combination);
+int hom_plan_destroy(hom_plan *plan);
+
+/* Device residency of HO_Phantom fields (reference src/datatypes/HO_Phantom.jl:47-71;
+ * upload step src/simulation/Simulate.jl:303-309).  csm is Ns x Nc column-major
+ * complex double with column stride ld (>= Ns). */
+int hom_upload_phantom(hom_plan *plan, const double *x, const double *y, const double *z,
+                       const double *rho, const double *T1, const double *T2, const double *dw);
+int hom_upload_csm(hom_plan *plan, const double *csm, int64_t ld);
+int hom_upload_sequence(hom_plan *plan, const hom_sequence_host *seq);
+
+/* Spin state Mag(xy, z).  reset: Mxy = 0, Mz = rho (reference
+ * src/simulation/Bloch/BlochHighOrderSimulatitonMethod.jl:9-15).  get/set
+ * copy Ns complex double Mxy and Ns double Mz (synchronous). */
+int hom_reset_state(hom_plan *plan);
+int hom_set_state(hom_plan *plan, const double *mxy, const double *mz);
+int hom_get_state(hom_plan *plan, double *mxy, double *mz);
+
+/* Per-block drop-ins for run_high_order_precession! / run_high_order_excitation!
+ * (reference src/simulation/kernels/HighOrderGPU.jl:124-140 and :223-233):
+ * evolve the state over nodes [first, last] and write the block's
+ * Nadc x Nc signal (overwrite, transpose CSM, no conjugation) into device
+ * memory out_dev of the plan's signal type with column stride out_ld.
+ * out_dev may be NULL when the block has no ADC sample.  Stream-ordered: the
+ * call returns before the device finishes. */
+int hom_precession_block(hom_plan *plan, int64_t first, int64_t last, int flags, void *out_dev,
+                         int64_t out_ld);
+int hom_excitation_block(hom_plan *plan, int64_t first, int64_t last, int flags, void *out_dev,
+                         int64_t out_ld);
+
+/* reduce_high_order_signal! (reference src/simulation/kernels/CoilReductionGPU.jl:79-100):
+ * deterministic two-stage coil weighting and spin reduction of a device state
+ * matrix (Ns x n_adc, column stride ld_states, plan's complex type) with the
+ * plan's CSM into out_dev (n_adc x Nc, column stride out_ld). */
+int hom_reduce_signal(hom_plan *plan, const void *states_dev, int64_t ld_states, int64_t n_adc,
+                      void *out_dev, int64_t out_ld);
+
+/* Whole block loop (reference src/simulation/SimulatorCore.jl:96-152) over
+ * the given parts, from the current state, writing sum(ADC) x Nc rows into the
+ * plan's signal buffer.  HOM_F32/HOM_F64 plans run the per-block kernels;
+ * HOM_MIXED plans fuse consecutive precession blocks into segments. */
+int hom_run(hom_plan *plan, const int64_t *part_first, const int64_t *part_last,
+            const uint8_t *part_is_rf, int64_t n_parts, int flags);
+int hom_synchronize(hom_plan *plan);
+int hom_signal_rows(hom_plan *plan, int64_t *rows);
+/* Copies the signal buffer (rows x Nc, column stride ld on the host). */
+int hom_download_signal(hom_plan *plan, void *host_out, int64_t ld);
+/* Element size in bytes of one complex signal value (8 or 16). */
+int hom_signal_element_size(hom_plan *plan, int64_t *bytes);
+
+/* Device buffers for hosts without a GPU array library of their own. */
+int hom_buffer_alloc(hom_plan *plan, int64_t bytes, void **dev);
+int hom_buffer_free(hom_plan *plan, void *dev);
+int hom_buffer_upload(hom_plan *plan, void *dev, const void *host, int64_t bytes);
+int hom_buffer_download(hom_plan *plan, void *host, const void *dev, int64_t bytes);
+
+/* Per-kernel timing with HIP events on the plan's stream (off by default). */
+int hom_plan_set_timing(hom_plan *plan, int enable);
+int hom_plan_kernel_time(hom_plan *plan, int kernel_id, double *total_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,99 @@
+"""CPU-side checks of the boundary: the C ABI library loads and exports every
+symbol include/hom_abi.h declares, the host mirror validates like the
+reference (src/simulation/BlochHighOrder.jl:21-43, test/unit/test_phantom_interfaces.jl:88-103),
+and invalid shapes are rejected before any device work."""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import hom_amd
+from hom_amd import runtime as rt
+from hom_amd.types import BlochHighOrder, DimensionMismatch, HO_DiscreteSequence, HO_Phantom
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def header_symbols():
+    text = (ROOT / "include" / "hom_abi.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(hom_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = rt.load_library()
+    declared = header_symbols()
+    assert len(declared) >= 20
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert set(declared) == set(rt.ABI_SYMBOLS)
+    assert lib.hom_abi_version() == 1
+
+
+def test_plan_create_without_device_or_bad_args_reports_error():
+    lib = rt.load_library()
+    # Argument checks run before any device query.
+    with pytest.raises(ValueError, match="maximum ADC"):
+        rt.Plan("f64", 4, 1, 3, -1)
+    with pytest.raises(ValueError, match="RF ADC"):
+        rt.Plan("f64", 4, 1, 3, 2, max_rf_adc=3)
+    with pytest.raises(ValueError, match="coil_combination"):
+        rt.Plan("f64", 4, 1, 3, 2, coil_combination="magic")
+    with pytest.raises(ValueError):
+        rt.Plan("f16", 4, 1, 3, 2)
+    if rt.device_count() == 0:
+        with pytest.raises(rt.HomDeviceError, match="No HIP device"):
+            rt.Plan("f64", 4, 1, 3, 2)
+    assert lib is rt.load_library()
+
+
+def test_mask_consistency():
+    for bits in range(16):
+        name = format(bits, "04b")
+        flags = tuple(c == "1" for c in name)
+        for m in (BlochHighOrder(name), BlochHighOrder(name, False, True)):
+            assert (m.ho0, m.ho1, m.ho2, m.ho3) == flags
+            assert m.name == name
+    for bad in ("", "000", "00000", "10x1", "111₂"):
+        with pytest.raises(ValueError):
+            BlochHighOrder(bad)
+        with pytest.raises(ValueError):
+            BlochHighOrder(bad, False, False)
+    with pytest.raises(ValueError):
+        BlochHighOrder("1111", ho0=False)
+    assert BlochHighOrder(ho0=False).name == "0111"
+    m = BlochHighOrder("1010", False, True)
+    assert m.flags == (1 | 4 | 32)
+
+
+def test_phantom_defaults_and_validation():
+    p = HO_Phantom(x=[0.0, 0.01])
+    assert np.all(p.rho == 1) and np.all(p.T1 == 1e6) and np.all(p.T2 == 1e6)
+    assert p.csm.shape == (2, 1) and np.all(p.csm == 1) and p.csm.flags.f_contiguous
+    p.csm = np.ones((3, 1), complex)
+    with pytest.raises(DimensionMismatch):
+        p.validate()
+
+
+def test_sequence_validation():
+    n = 4
+    ok = dict(t=np.arange(n) * 1e-3, gx=np.zeros(n), gy=np.zeros(n), gz=np.zeros(n),
+              b1=np.zeros(n), df=np.zeros(n), psi=np.zeros(n), adc=np.ones(n, bool),
+              excitation=np.zeros(n - 1, bool), h=np.zeros((16, n)))
+    s = HO_DiscreteSequence(**ok)
+    assert len(s) == 3 and s.n_nodes == 4
+    assert s.block_adc(0, 3) == 3
+    v = s.view(1, 3)
+    assert v.n_nodes == 3 and len(v) == 2
+    with pytest.raises(DimensionMismatch):
+        HO_DiscreteSequence(**{**ok, "h": np.zeros((15, n))})
+    with pytest.raises(DimensionMismatch):
+        HO_DiscreteSequence(**{**ok, "excitation": np.zeros(n, bool)})
+    with pytest.raises(ValueError):
+        HO_DiscreteSequence(**{**ok, "t": np.array([0, 1e-3, 0.5e-3, 2e-3])})
+
+
+def test_package_surface():
+    for name in ("BlochHighOrder", "HO_Phantom", "HO_DiscreteSequence", "get_sim_ranges", "GAMMA"):
+        assert hasattr(hom_amd, name)
