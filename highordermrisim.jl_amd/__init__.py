@@ -18,3 +18,9 @@ from . import workloads  # noqa: F401
 
 __all__ = ["GAMMA", "BlochHighOrder", "DimensionMismatch", "HO_DiscreteSequence", "HO_Phantom",
            "get_sim_ranges", "workloads"]
+
+from .simulate import (Mag, PreparedSimulation, default_sim_params, prepare_simulation,  # noqa: E402,F401
+                       setup_simulation, simulate)
+
+__all__ += ["Mag", "PreparedSimulation", "default_sim_params", "prepare_simulation",
+            "setup_simulation", "simulate"]

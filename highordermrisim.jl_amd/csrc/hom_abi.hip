@@ -262,6 +262,7 @@ int hom_upload_phantom(hom_plan *p, const double *x, const double *y, const doub
     rc = rc ? rc : upload_real(p, &p->d.dw, dw, p->ns);
     if (rc) return rc;
     p->has_phantom = true;
+    if (p->mixed) mixed_free(p);  // cached operands depend on the phantom
     return hom_reset_state(p);
 }
 

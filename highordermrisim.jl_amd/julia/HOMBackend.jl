@@ -1,0 +1,220 @@
+# HOMBackend.jl -- Julia ccall shim over libhom.so (include/hom_abi.h).
+#
+# Included from HighOrderMRISim's module root (src/HighOrderMRISim.jl:21-41),
+# it replaces the CUDA kernel internals of src/simulation/kernels/*.jl behind
+# the unchanged simulate(obj, hoseq, sys; sim_params) API:
+#
+#   HighOrderGPUWorkspace(...)            -> HOMWorkspace(...)        (HighOrderGPU.jl:16-58)
+#   run_high_order_precession!(ws, ...)   -> hom_precession_block     (HighOrderGPU.jl:124-140)
+#   run_high_order_excitation!(ws, ...)   -> hom_excitation_block     (HighOrderGPU.jl:223-233)
+#   run_sim_time_iter! without callbacks  -> hom_run                  (SimulatorCore.jl:96-152)
+#
+# sim_params keep their meaning: "precision" => "f32" | "f64" | "mixed",
+# "coil_combination", "return_type", "max_block_length", ...  Host arrays are
+# passed as plain pointers in Julia's own layouts (column-major, interleaved
+# Complex{Float64}); BitVector flags are widened to UInt8 first.
+module HOMBackend
+
+export HOMWorkspace, hom_simulate_shard, hom_available
+
+const libhom = get(ENV, "HOM_LIBRARY",
+    joinpath(@__DIR__, "..", "libhom.so"))
+
+const HOM_OK = Cint(0)
+const PRECISIONS = Dict("f32" => Cint(0), "f64" => Cint(1), "mixed" => Cint(2))
+const COIL_MODES = Dict("gemm" => Cint(0), "reduction" => Cint(1),
+                        "fused_warp" => Cint(2), "fused_tiled" => Cint(3))
+
+struct HOMSequenceHost
+    n_nodes::Int64
+    t::Ptr{Float64}
+    dt::Ptr{Float64}
+    gx::Ptr{Float64}
+    gy::Ptr{Float64}
+    gz::Ptr{Float64}
+    b1::Ptr{Float64}
+    df::Ptr{Float64}
+    psi::Ptr{Float64}
+    adc::Ptr{UInt8}
+    excitation::Ptr{UInt8}
+    h::NTuple{16,Ptr{Float64}}
+end
+
+hom_available() = isfile(libhom)
+
+last_error(plan::Ptr{Cvoid}) = unsafe_string(ccall((:hom_last_error, libhom), Cstring, (Ptr{Cvoid},), plan))
+
+# Status -> the reference's exception types (src/simulation/Simulate.jl:226-278 conventions).
+function check(status::Cint, plan::Ptr{Cvoid})
+    status == HOM_OK && return nothing
+    msg = last_error(plan)
+    status == Cint(1) && throw(ArgumentError(msg))
+    status == Cint(2) && throw(DimensionMismatch(msg))
+    error(msg)
+end
+
+mutable struct HOMWorkspace
+    handle::Ptr{Cvoid}
+    precision::String
+    nspins::Int
+    ncoils::Int
+    nnodes::Int
+    keep::Vector{Any}          # host arrays pinned for the duration of a call
+    function HOMWorkspace(handle, precision, nspins, ncoils, nnodes)
+        ws = new(handle, precision, nspins, ncoils, nnodes, Any[])
+        finalizer(close!, ws)
+        return ws
+    end
+end
+
+function close!(ws::HOMWorkspace)
+    if ws.handle != C_NULL
+        ccall((:hom_plan_destroy, libhom), Cint, (Ptr{Cvoid},), ws.handle)
+        ws.handle = C_NULL
+    end
+    return nothing
+end
+
+f64vec(x) = Vector{Float64}(x)
+u8vec(x) = Vector{UInt8}(collect(x))
+
+"""
+    HOMWorkspace(obj, hoseqd, max_adc; precision="f64", coil_combination="gemm",
+                 max_rf_adc=max_adc, device=0, gamma=γ)
+
+Device-resident plan for one phantom and one sampled sequence: the same
+argument checks as HighOrderGPUWorkspace (HighOrderGPU.jl:19-35), plus the
+upload of the phantom, CSM and sequence (Simulate.jl:303-309).
+"""
+function HOMWorkspace(obj, hoseqd, max_adc; precision="f64", coil_combination="gemm",
+                      max_rf_adc=max_adc, device=0, gamma)
+    haskey(PRECISIONS, precision) || throw(ArgumentError("The precision must be f32, f64 or mixed."))
+    haskey(COIL_MODES, coil_combination) ||
+        throw(ArgumentError("Unknown coil_combination: $coil_combination."))
+    seqd = hoseqd.seqd
+    ns, nc, nn = length(obj), size(obj.csm, 2), length(seqd.t)
+    handle = Ref{Ptr{Cvoid}}(C_NULL)
+    st = ccall((:hom_plan_create, libhom), Cint,
+        (Ptr{Ptr{Cvoid}}, Cint, Cint, Int64, Int64, Int64, Int64, Int64, Float64, Cint),
+        handle, device, PRECISIONS[precision], ns, nc, nn, max_adc, max_rf_adc, Float64(gamma),
+        COIL_MODES[coil_combination])
+    check(st, C_NULL)
+    ws = HOMWorkspace(handle[], precision, ns, nc, nn)
+    upload_phantom!(ws, obj)
+    upload_sequence!(ws, hoseqd)
+    return ws
+end
+
+function upload_phantom!(ws::HOMWorkspace, obj)
+    x, y, z, ρ, T1, T2, Δw = map(f64vec, (obj.x, obj.y, obj.z, obj.ρ, obj.T1, obj.T2, obj.Δw))
+    check(ccall((:hom_upload_phantom, libhom), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+         Ptr{Float64}, Ptr{Float64}), ws.handle, x, y, z, ρ, T1, T2, Δw), ws.handle)
+    csm = Matrix{ComplexF64}(obj.csm)     # dense column-major copy of a possibly strided view
+    check(ccall((:hom_upload_csm, libhom), Cint, (Ptr{Cvoid}, Ptr{ComplexF64}, Int64),
+        ws.handle, csm, max(size(csm, 1), 1)), ws.handle)
+    return ws
+end
+
+function upload_sequence!(ws::HOMWorkspace, hoseqd)
+    s = hoseqd.seqd
+    arrays = (f64vec(s.t), f64vec(s.Δt), f64vec(s.Gx), f64vec(s.Gy), f64vec(s.Gz),
+              Vector{ComplexF64}(s.B1), f64vec(s.Δf), f64vec(s.ψ))
+    adc, exc = u8vec(s.ADC), u8vec(s.excitation_bool)
+    fields = ntuple(j -> f64vec(getfield(hoseqd, j + 1)), 16)
+    GC.@preserve arrays adc exc fields begin
+        host = HOMSequenceHost(length(s.t), pointer(arrays[1]), pointer(arrays[2]),
+            pointer(arrays[3]), pointer(arrays[4]), pointer(arrays[5]),
+            Ptr{Float64}(pointer(arrays[6])), pointer(arrays[7]), pointer(arrays[8]),
+            pointer(adc), pointer(exc), ntuple(j -> pointer(fields[j]), 16))
+        check(ccall((:hom_upload_sequence, libhom), Cint, (Ptr{Cvoid}, Ref{HOMSequenceHost}),
+            ws.handle, host), ws.handle)
+    end
+    return ws
+end
+
+method_flags(m) = Cint((m.ho0 ? 1 : 0) | (m.ho1 ? 2 : 0) | (m.ho2 ? 4 : 0) | (m.ho3 ? 8 : 0) |
+                       (m.Δw_excitation ? 16 : 0) | (m.Δw_precession ? 32 : 0))
+
+function set_state!(ws::HOMWorkspace, M)
+    xy, z = Vector{ComplexF64}(M.xy), f64vec(M.z)
+    check(ccall((:hom_set_state, libhom), Cint, (Ptr{Cvoid}, Ptr{ComplexF64}, Ptr{Float64}),
+        ws.handle, xy, z), ws.handle)
+end
+
+function get_state!(M, ws::HOMWorkspace)
+    xy, z = zeros(ComplexF64, ws.nspins), zeros(Float64, ws.nspins)
+    check(ccall((:hom_get_state, libhom), Cint, (Ptr{Cvoid}, Ptr{ComplexF64}, Ptr{Float64}),
+        ws.handle, xy, z), ws.handle)
+    M.xy .= xy
+    M.z .= z
+    return M
+end
+
+"""
+    hom_simulate_shard(obj, hoseqd, params, plan; callbacks=(), gamma)
+
+Drop-in for the kernel branch of simulate_shard (Simulate.jl:290-345): one
+device plan, the whole block loop through hom_run when there are no callbacks
+(mixed precision fuses precession blocks), per-block entry points with state
+round trips when callbacks must observe each block (SimulatorCore.jl:109-111).
+Returns (; signal, state) with the signal as (sum(ADC), Nc, 1).
+"""
+function hom_simulate_shard(obj, hoseqd, params, plan; callbacks=(), gamma, M)
+    (; parts, excitation_bool, adc_counts, max_adc, max_rf_adc) = plan
+    precision = get(params, "precision", "f64")
+    ws = HOMWorkspace(obj, hoseqd, max_adc; precision, max_rf_adc, gamma,
+                      coil_combination=get(params, "coil_combination", "gemm"))
+    try
+        set_state!(ws, M)
+        flags = method_flags(params["sim_method"])
+        pf = Int64[first(p) - 1 for p in parts]           # 0-based node ranges
+        pl = Int64[last(p) - 1 for p in parts]
+        rf = UInt8.(excitation_bool)
+        T = precision == "f32" ? ComplexF32 : ComplexF64
+        nc = size(obj.csm, 2)
+        total = sum(adc_counts)
+        if isempty(callbacks)
+            check(ccall((:hom_run, libhom), Cint,
+                (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{UInt8}, Int64, Cint),
+                ws.handle, pf, pl, rf, length(pf), flags), ws.handle)
+            sig = zeros(T, total, nc)
+            check(ccall((:hom_download_signal, libhom), Cint, (Ptr{Cvoid}, Ptr{T}, Int64),
+                ws.handle, sig, max(total, 1)), ws.handle)
+        else
+            sig = zeros(T, total, nc)
+            dev = Ref{Ptr{Cvoid}}(C_NULL)
+            bytes = sizeof(T) * total * nc
+            check(ccall((:hom_buffer_alloc, libhom), Cint, (Ptr{Cvoid}, Int64, Ptr{Ptr{Cvoid}}),
+                ws.handle, bytes, dev), ws.handle)
+            row = 0
+            for (b, p) in enumerate(parts)
+                fn = excitation_bool[b] ? :hom_excitation_block : :hom_precession_block
+                out = adc_counts[b] == 0 ? C_NULL : dev[] + sizeof(T) * row
+                st = excitation_bool[b] ?
+                    ccall((:hom_excitation_block, libhom), Cint,
+                        (Ptr{Cvoid}, Int64, Int64, Cint, Ptr{Cvoid}, Int64),
+                        ws.handle, pf[b], pl[b], flags, out, max(total, 1)) :
+                    ccall((:hom_precession_block, libhom), Cint,
+                        (Ptr{Cvoid}, Int64, Int64, Cint, Ptr{Cvoid}, Int64),
+                        ws.handle, pf[b], pl[b], flags, out, max(total, 1))
+                check(st, ws.handle)
+                row += adc_counts[b]
+                check(ccall((:hom_buffer_download, libhom), Cint,
+                    (Ptr{Cvoid}, Ptr{T}, Ptr{Cvoid}, Int64), ws.handle, sig, dev[], bytes), ws.handle)
+                get_state!(M, ws)
+                progress = (; block=b, Nblocks=length(parts), samples=row + 1)
+                for cb in callbacks
+                    cb(progress, (; excitation_bool, parts), (; obj, seqd=hoseqd, sig, Xt=M), params)
+                end
+                set_state!(ws, M)           # callbacks may edit the state
+            end
+        end
+        get_state!(M, ws)
+        return (; signal=reshape(sig, total, nc, 1), state=M)
+    finally
+        close!(ws)
+    end
+end
+
+end # module

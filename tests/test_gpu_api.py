@@ -1,0 +1,167 @@
+"""Public simulate / prepare_simulation surface on the GPU, against closed forms
+and the oracle.  Mirrors test/unit/test_koma_compatibility.jl:55-135,
+test/unit/test_phantom_construction.jl:123-150,
+test/gpu/test_gpu_compatibility.jl:61-230 and test/gpu/test_prepared_simulation.jl."""
+import numpy as np
+import pytest
+
+import hom_amd
+import oracle
+from hom_amd import BlochHighOrder, HO_Phantom, prepare_simulation, simulate
+from hom_amd.types import GAMMA, get_sim_ranges
+from hom_amd.workloads import FreeBlock, RFBlock, build_sequence
+from helpers import assert_isapprox, nrmse, print_case, require_gpu
+
+pytestmark = pytest.mark.gpu
+TAU = 1e-4
+TIMES = 1e-3 * np.arange(1, 6) / 5
+
+
+def compat(h_readout=None, csm=None, rho=(1.0, 0.7), x=(0.0, 0.01), dw=None, inf=True):
+    if csm is None:
+        csm = np.array([[1, 1j], [0.5 - 0.2j, -0.3 + 0.1j]])
+    obj = HO_Phantom(x=np.array(x), rho=np.array(rho), csm=csm, dw=dw,
+                     T1=np.full(len(x), np.inf) if inf else None,
+                     T2=np.full(len(x), np.inf) if inf else None)
+
+    def fields(t):
+        out = np.zeros((16, t.size))
+        for j, v in (h_readout or {}).items():
+            out[j, t >= TAU] = v
+        return out
+
+    seq = build_sequence([RFBlock(TAU, 1 / (4 * GAMMA * TAU)), FreeBlock(2e-3, TIMES)], fields)
+    return obj, seq
+
+
+def params(**kw):
+    p = {"precision": "f64", "sim_method": BlochHighOrder("1111"), "return_type": "mat"}
+    p.update(kw)
+    return p
+
+
+def test_legacy_api_state_ownership_and_kats():
+    require_gpu()
+    obj, seq = compat()
+    density = obj.rho.copy()
+    p = params(sim_method=BlochHighOrder("0000"), Nblocks=1)
+    original = dict(p)
+    sig = simulate(obj, seq, sim_params=p)
+    expected = 1j * (density @ obj.csm)
+    assert sig.shape == (5, 2, 1)
+    np.testing.assert_allclose(sig[:, :, 0], np.tile(expected, (5, 1)), atol=1e-12)
+    assert np.array_equal(obj.rho, density)
+    assert p == original
+    seen = []
+    split = simulate(obj, seq, sim_params={**p, "Nblocks": 1000},
+                     callbacks=[lambda prog, *_: seen.append(prog["samples"] - 1)])
+    np.testing.assert_allclose(split, sig, atol=1e-12)
+    assert seen == sorted(seen) and seen[-1] == 5
+    single = simulate(obj, seq, sim_params={**p, "precision": "f32"})
+    assert single.dtype == np.complex64
+    assert_isapprox(single, sig, 2e-5, 2e-5)
+    mixed = simulate(obj, seq, sim_params={**p, "precision": "mixed"})
+    assert nrmse(mixed, sig) <= 1e-5
+    state = simulate(obj, seq, sim_params={**p, "return_type": "state"})
+    np.testing.assert_allclose(state.xy, 1j * density, atol=1e-12)
+    raw = simulate(obj, seq, sim_params={**p, "return_type": "raw"})
+    assert raw["signal"].shape == (5, 2, 1) and raw["params"]["Nblocks"] >= 2
+
+
+def test_zeroth_order_phase_and_rf_ignores_fields():
+    require_gpu()
+    obj, seq = compat(h_readout={0: 20 / GAMMA})
+    for precision in ("f64", "mixed"):
+        sig = simulate(obj, seq, sim_params=params(sim_method=BlochHighOrder("1000"), precision=precision))
+        expected = 1j * (obj.rho @ obj.csm)
+        ref = np.exp(-1j * 2 * np.pi * 20 * TIMES)[:, None] * expected
+        if precision == "f64":
+            np.testing.assert_allclose(sig[:, :, 0], ref, atol=1e-12)
+        else:
+            assert nrmse(sig[:, :, 0], ref) <= 1e-5
+
+
+def test_ideal_receive_coil_and_weights():
+    require_gpu()
+    for precision, tol in (("f32", 20 * np.finfo(np.float32).eps), ("f64", 20 * np.finfo(float).eps)):
+        print_case("ideal-coil", precision)
+        obj, seq = compat(csm=np.ones((2, 1)))
+        sig = simulate(obj, seq, sim_params=params(precision=precision))
+        np.testing.assert_allclose(np.abs(sig[:, 0, 0]), obj.rho.sum(), rtol=tol)
+        w = np.array([[0, 2 + 1j], [0, 2 + 1j]])
+        objw, seqw = compat(csm=w)
+        res = simulate(objw, seqw, sim_params=params(precision=precision))
+        assert np.all(res[:, 0, 0] == 0)
+        np.testing.assert_allclose(res[:, 1, 0], (2 + 1j) * sig[:, 0, 0], rtol=tol)
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64", "mixed"])
+def test_high_order_compatibility_all_coil_modes(precision):
+    """test_gpu_compatibility.jl:125-171: measured fields on every term, off-resonance,
+    masks 0000/1000/1111, every coil_combination, block lengths 2/1."""
+    require_gpu()
+    for mask in ("0000", "1000", "1111"):
+        obj, seq = compat(h_readout={j: (j + 1) * 1e-7 for j in range(16)}, dw=np.array([5.0, 25.0]),
+                          inf=True)
+        m = BlochHighOrder(mask)
+        parts = get_sim_ranges(seq, 2, 1)
+        ref, _, _ = oracle.run(obj, seq, parts, m.flags, GAMMA)
+        for mode in ("gemm", "reduction", "fused_warp", "fused_tiled"):
+            print_case("compat", precision, mask, mode)
+            count = [0]
+            sig = simulate(obj, seq, sim_params=params(precision=precision, sim_method=m,
+                                                       coil_combination=mode, max_block_length=2,
+                                                       max_rf_block_length=1),
+                           callbacks=[lambda *a: count.__setitem__(0, count[0] + 1)])
+            assert count[0] > 0
+            if precision == "mixed":
+                assert nrmse(sig[:, :, 0], ref) <= 1e-5
+            else:
+                t = 3e-5 if precision == "f32" else 1e-11
+                assert_isapprox(sig[:, :, 0], ref, t, t)
+            assert np.all(np.isfinite(sig))
+
+
+def test_block_length_invariance_public():
+    require_gpu()
+    obj, seq = compat(h_readout={j: (j + 1) * 1e-7 for j in range(16)})
+    ref = simulate(obj, seq, sim_params=params(max_block_length=1))
+    for L in (2, 32, 128):
+        sig = simulate(obj, seq, sim_params=params(max_block_length=L))
+        assert_isapprox(sig, ref, 1e-11, 1e-11)
+
+
+def test_prepared_simulation_reuse_and_failures():
+    require_gpu()
+    obj, seq = compat(h_readout={j: (j + 1) * 1e-7 for j in range(16)}, inf=False)
+    one_shot = simulate(obj, seq, sim_params=params())
+    plan = prepare_simulation(obj, seq, sim_params=params())
+    try:
+        a = plan.simulate()
+        b = plan.simulate()
+        np.testing.assert_array_equal(a, b)
+        assert_isapprox(a, one_shot, 1e-12, 1e-12)
+
+        def failing(*_):
+            raise RuntimeError("callback failure")
+        with pytest.raises(RuntimeError, match="callback failure"):
+            plan.simulate(callbacks=[failing])
+        c = plan.simulate()
+        np.testing.assert_array_equal(a, c)
+
+        def recurse(*_):
+            plan.simulate()
+        with pytest.raises(ValueError, match="own callback"):
+            plan.simulate(callbacks=[recurse])
+
+        def zero_state(progress, blocks, data, p):
+            if progress["block"] == 1:
+                data["Xt"].xy[:] = 0
+                data["Xt"].z[:] = 0
+        edited = plan.simulate(callbacks=[zero_state])
+        assert np.abs(edited).max() < np.abs(a).max()
+    finally:
+        plan.close()
+        plan.close()
+    with pytest.raises(ValueError):
+        plan.simulate()
