@@ -261,6 +261,10 @@ int hom_upload_phantom(hom_plan *p, const double *x, const double *y, const doub
     rc = rc ? rc : upload_real(p, &p->d.T2, T2, p->ns);
     rc = rc ? rc : upload_real(p, &p->d.dw, dw, p->ns);
     if (rc) return rc;
+    for (int j = 0; j <= 16; ++j) p->h_basis_max[j] = 0.0;
+    for (int64_t s = 0; s < p->ns; ++s)
+        for (int j = 1; j <= 16; ++j)
+            p->h_basis_max[j] = std::max(p->h_basis_max[j], std::fabs(basis_term(j, x[s], y[s], z[s], dw[s])));
     p->has_phantom = true;
     if (p->mixed) mixed_free(p);  // cached operands depend on the phantom
     return hom_reset_state(p);

@@ -13,6 +13,32 @@
 namespace hom {
 
 constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr int kTermDW = 16;  // term code of the off-resonance term
+
+// Unnormalised real solid harmonic j (1..15) at (x, y, z), in the reference's
+// order (src/simulation/kernels/HighOrderGPU.jl:63-71); code kTermDW returns dw.
+__host__ __device__ inline double basis_term(int code, double x, double y, double z, double dw) {
+    const double x2 = x * x, y2 = y * y, z2 = z * z, r2 = x2 + y2 + z2;
+    switch (code) {
+        case 1: return x;
+        case 2: return y;
+        case 3: return z;
+        case 4: return x * y;
+        case 5: return z * y;
+        case 6: return 3.0 * z2 - r2;
+        case 7: return x * z;
+        case 8: return x2 - y2;
+        case 9: return 3.0 * y * x2 - y * y2;
+        case 10: return x * z * y;
+        case 11: return (5.0 * z2 - r2) * y;
+        case 12: return 5.0 * z * z2 - 3.0 * z * r2;
+        case 13: return (5.0 * z2 - r2) * x;
+        case 14: return x2 * z - y2 * z;
+        case 15: return x * x2 - 3.0 * x * y2;
+        case kTermDW: return dw;
+        default: return 0.0;
+    }
+}
 
 // Device arrays of one precision (float for HOM_F32, double otherwise).
 struct DeviceArrays {
@@ -55,6 +81,7 @@ struct hom_plan {
     std::vector<double> h_t, h_dt, h_gx, h_gy, h_gz, h_coef;  // coef: 16 x nn
     std::vector<uint8_t> h_adc, h_exc;
     std::vector<double> h_csm_colmax;  // max |csm| per coil
+    double h_basis_max[17] = {};       // max over spins of |basis term j| (j = 1..16)
 
     void *signal = nullptr;
     int64_t signal_rows = 0, signal_capacity_rows = 0;

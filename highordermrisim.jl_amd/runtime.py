@@ -12,13 +12,14 @@ is missing, and nothing here falls back to a CPU path.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
 from .types import GAMMA, BlochHighOrder, DimensionMismatch, HO_DiscreteSequence, HO_Phantom
 
-LIB_PATH = Path(__file__).resolve().parent / "libhom.so"
+LIB_PATH = Path(os.environ.get("HOM_LIBRARY") or Path(__file__).resolve().parent / "libhom.so")
 
 HOM_OK, HOM_ERR_ARGUMENT, HOM_ERR_DIMENSION, HOM_ERR_DEVICE, HOM_ERR_MEMORY, HOM_ERR_STATE = range(6)
 PRECISIONS = {"f32": 0, "f64": 1, "mixed": 2}
