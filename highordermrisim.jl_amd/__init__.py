@@ -24,3 +24,7 @@ from .simulate import (Mag, PreparedSimulation, default_sim_params, prepare_simu
 
 __all__ += ["Mag", "PreparedSimulation", "default_sim_params", "prepare_simulation",
             "setup_simulation", "simulate"]
+
+from . import serving  # noqa: E402,F401
+
+__all__ += ["serving"]

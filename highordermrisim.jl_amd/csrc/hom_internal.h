@@ -71,6 +71,7 @@ struct hom_plan {
     int precision = HOM_F64;
     int coil_mode = HOM_COIL_GEMM;
     int64_t ns = 0, nc = 0, nn = 0, max_adc = 0, max_rf_adc = 0;
+    int64_t csm_ld = 0;  // column stride of d.csm (0: ns)
     double gamma = 0.0;
     hipStream_t stream = nullptr;
     std::string err;

@@ -1,0 +1,701 @@
+// mixed_common.h -- types, host table builder and device bodies of the
+// segment-fused mixed-precision BlochHighOrder path, shared by the per-plan
+// launches (mixed_kernels.hip) and the batched serving steps (serve.hip).
+//
+// Math (restating src/simulation/Bloch/BlochHighOrderSimulatitonMethod.jl:43-94
+// and src/simulation/kernels/HighOrderGPU.jl:95-122 in integrated form):
+// the trapezoid rule is linear, so the phase of spin s at node n of a
+// free-precession segment starting at node n0 is
+//     phi(s, n) / 2pi = arg(M0_s)/2pi + sum_j b_j(s) * Kr_j[n] + Kr_0[n]
+//     Kr_j[n] = -gamma * sum_{m=n0}^{n-1} (h_j[m] + h_j[m+1]) dt_m / 2
+// with b_j the masked spatial basis (nominal gradients in place of h1..h3 when
+// order one is off) and an extra term b = dw, Kr = -t/2pi for off-resonance.
+// Consecutive precession blocks therefore fuse into one segment; only RF
+// blocks are sequential barriers.
+//
+// Phase arithmetic.  The first-order and off-resonance terms (x, y, z, dw)
+// carry almost all of the phase (tens of revolutions) and run in FP64 on the
+// FP64 matrix pipe, one v_mfma_f64_16x16x4 per 16 spins x 16 samples.  The up
+// to 12 higher-order terms are split at anchor blocks of 256 samples: their
+// value at the block's first sample (the anchor) is computed per spin in FP64
+// and seeds the FP64 accumulator together with arg(M0); only the small
+// offsets from the anchor go through split-FP16 MFMAs (hi*hi + lo*hi + hi*lo
+// stacked along K), with FP32 accumulation.  FP64 MFMAs do not co-issue with
+// VALU work on gfx950; FP16 MFMAs do, so this moves three quarters of the
+// phase work onto the pipe that overlaps.  When a segment's offsets would not
+// fit FP16, the segment falls back to the all-FP64 phase body (main64_body).
+//
+// MFMA lane layouts (measured on gfx950): f16 16x16x32 -- A[l%16][8(l/16)+i],
+// B[8(l/16)+i][l%16], D[4(l/16)+v][l%16]; f64 16x16x4 -- A[l%16][l/16],
+// B[l/16][l%16], D[l/16 + 4v][l%16].  Lane l therefore holds FP64 phases of
+// spins g + 4v (g = l/16) at sample l%16; every spin-side array is stored in
+// that order, and FP16 A operands use row m <-> spin m/4 + 4(m%4) so both
+// outputs line up in registers.
+#pragma once
+
+#include "hom_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace hom {
+
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+// Both coil-GEMM operands are pre-scaled by 2^kOpScale (|values| <= 2^14 <
+// FP16 max) so the low halves of their FP16 splits stay in the normal range.
+constexpr int kOpScale = 14;
+// Higher-order phase offsets: A = b_j * 2^a_j with max |A| in (2^9, 2^10],
+// B = dKr_j * 2^(kHOOut - a_j); the FP32 product sum is offset * 2^kHOOut.
+constexpr int kHOA = 10;
+constexpr int kHOOut = 20;
+// |B| <= 2^9 with |A| <= 2^10 bounds every offset by 0.5 revolutions, so the
+// offset can be added to the reduced FP64 phase in FP32 (error <= 1e-7 rev).
+// Segments with larger offsets use the all-FP64 phase body.
+constexpr float kHOBMax = 512.0f;
+// Instruction interleave of the per-group compute: after the tile loop the
+// scheduler is asked to alternate one MFMA with HOM_SCHED_VALU VALU
+// instructions (0 leaves the default schedule).
+#ifndef HOM_SCHED_VALU
+#define HOM_SCHED_VALU 0
+#endif
+#ifndef HOM_TILES
+#define HOM_TILES 2
+#endif
+#ifndef HOM_WAVES_PER_EU
+#define HOM_WAVES_PER_EU 3
+#endif
+constexpr int kWaves = 4;                      // waves per workgroup
+constexpr int kTiles = HOM_TILES;              // 16-sample tiles per wave
+constexpr int kBlockTiles = kWaves * kTiles;   // 16-sample tiles per workgroup (sample block)
+// Higher-order anchors every kAnchorTiles tiles (256 samples); a workgroup's
+// sample block always lies inside one anchor block.
+constexpr int kAnchorTiles = 16;
+constexpr int kBlocksPerAnchor = kAnchorTiles / kBlockTiles;
+static_assert(kAnchorTiles % kBlockTiles == 0, "anchor blocks must hold whole sample blocks");
+
+// Active spatial terms of a field-order mask, in packing order.
+struct TermList {
+    int n;
+    int code[16];  // 1..15: basis index j; kTermDW: off-resonance
+};
+
+// The same terms split into the FP64 group (x, y, z, dw) and higher orders.
+struct PhaseSplit {
+    int code64[4];   // 1, 2, 3, kTermDW or 0
+    int nho;         // number of higher-order terms (0, 5, 7 or 12)
+    int nmh;         // FP16 MFMAs per tile for the offsets: ceil(3 nho / 32)
+    int codeho[12];
+    int aexp[12];    // A-side power-of-two exponent per higher-order term
+};
+
+struct Segment {
+    bool rf = false;
+    bool fallback = false;         // all-FP64 phase for this segment
+    int64_t first = 0, last = 0;   // node range
+    int64_t nadc = 0, row0 = 0;    // ADC samples, first signal row
+    int64_t tiles = 0, nsb = 0;    // 16-sample tiles, sample blocks
+    int64_t nab = 0;               // anchor blocks
+    int64_t kr_off = 0, k64_off = 0, bho_off = 0, s_off = 0, kend_off = 0, anc_off = 0;
+    double T = 0.0;                // duration
+};
+
+// Host view of a sequence's node arrays (coef: 16 x nn, row j contiguous).
+struct SeqHost {
+    int64_t nn = 0;
+    double gamma = 0.0;
+    const double *dt = nullptr, *gx = nullptr, *gy = nullptr, *gz = nullptr, *coef = nullptr;
+    const uint8_t *adc = nullptr;
+};
+
+// Host FP64 tables of one run, concatenated over its segments.
+struct TableSet {
+    std::vector<double> kr, k64, kr0, kend, anc;
+    std::vector<half8_t> bho;
+    std::vector<float> tk;
+};
+
+TermList make_terms(int flags);
+PhaseSplit make_split(int flags, const double *basis_max);
+// Segments (consecutive precession parts fused) and their integrated tables.
+void build_tables(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, int NI, int flags,
+                  const int64_t *pf, const int64_t *pl, const uint8_t *rf, int64_t np,
+                  std::vector<Segment> &segs, TableSet &ts);
+
+// ---------------------------------------------------------------- device --
+
+// Power-of-two column scale >= m (1 when the column is all zero).
+__device__ __host__ inline double col_scale(float m) {
+    if (!(m > 0.0f)) return 1.0;
+    int e = 0;
+    frexp((double)m, &e);
+    return ldexp(1.0, e);
+}
+
+// Spin held by FP16-MFMA A row m: the row order that makes D[4g+v] line up
+// with the FP64 output row g + 4v.
+__device__ inline int row_spin(int m) { return (m >> 2) + 4 * (m & 3); }
+
+// Spin of MFMA output-row slot r (0..15) of a 16-spin group.
+__device__ inline int slot_spin(int r) { return r / 4 + 4 * (r % 4); }
+
+// Makes v an opaque, already rounded register value: the compiler may not
+// fold the arithmetic that produced it into later conversions.  Without this
+// the backend evaluates (half)(a*b) once as round16(round32(a*b)) and once as
+// round16(a*b) (v_fma_mix), and the two can differ by one FP16 ulp.
+__device__ inline float opaque(float v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+__device__ inline half2_t opaque2(half2_t v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+// x ~= hi + lo with hi = fp16(x) and lo = fp16(x - hi), both from one value.
+__device__ inline void split16(double v, _Float16 &hi, _Float16 &lo) {
+    const float vf = opaque((float)v);
+    hi = (_Float16)vf;
+    lo = (_Float16)(float)(v - (double)hi);
+}
+
+// ---- spin-side operands (one element per call; idx ranges in comments) ----
+
+// idx < ngroups * NI * 64: FP64 A operand of all terms (all-FP64 fallback).
+__device__ inline void basisA_elem(int64_t idx, int64_t ns, int NI, const TermList &tl,
+                                   const double *x, const double *y, const double *z,
+                                   const double *dw, double *basisA) {
+    const int lane = (int)(idx % 64);
+    const int64_t gi = idx / 64;
+    const int i = (int)(gi % NI);
+    const int64_t grp = gi / NI;
+    const int64_t s = grp * 16 + lane % 16;   // A row = spin
+    const int term = 4 * i + lane / 16;       // A column = term
+    double v = 0.0;
+    if (s < ns && term < tl.n) v = basis_term(tl.code[term], x[s], y[s], z[s], dw[s]);
+    basisA[idx] = v;
+}
+
+// idx < ngroups * 64: FP64 A operand of (x, y, z, dw).
+__device__ inline void basis64_elem(int64_t idx, int64_t ns, const PhaseSplit &ps, const double *x,
+                                    const double *y, const double *z, const double *dw,
+                                    double *basis64) {
+    const int lane = (int)(idx % 64);
+    const int64_t s = (idx / 64) * 16 + lane % 16;
+    const int code = ps.code64[lane / 16];
+    basis64[idx] = (s < ns && code) ? basis_term(code, x[s], y[s], z[s], dw[s]) : 0.0;
+}
+
+// idx < ngroups * nmh * 64: FP16 A operand of the offset MFMAs.  K slot q
+// (0 <= q < 3 nho) holds term q % nho; slots [0, nho) and [2 nho, 3 nho) use
+// the high half of the scaled basis value, slots [nho, 2 nho) the low half
+// (paired with hi / hi / lo halves of the offsets on the B side).
+__device__ inline void aho_elem(int64_t idx, int64_t ns, const PhaseSplit &ps, const double *x,
+                                const double *y, const double *z, const double *dw, half8_t *aho) {
+    const int lane = (int)(idx % 64);
+    const int64_t t = idx / 64;
+    const int m = (int)(t % ps.nmh);
+    const int64_t grp = t / ps.nmh;
+    const int64_t s = grp * 16 + row_spin(lane % 16);
+    half8_t out;
+    for (int i = 0; i < 8; ++i) {
+        const int q = 32 * m + 8 * (lane / 16) + i;
+        _Float16 v = (_Float16)0.0f;
+        if (s < ns && q < 3 * ps.nho) {
+            const int term = q % ps.nho, kind = q / ps.nho;
+            const double b = ldexp(basis_term(ps.codeho[term], x[s], y[s], z[s], dw[s]), ps.aexp[term]);
+            _Float16 hi, lo;
+            split16(b, hi, lo);
+            v = kind == 1 ? lo : hi;
+        }
+        out[i] = v;
+    }
+    aho[idx] = out;
+}
+
+// idx < ngroups * 16: cdec[grp*16 + 4g + v] = -log2(e) / T2 of spin grp*16 + g + 4v.
+__device__ inline void cdec_elem(int64_t idx, int64_t ns, const double *T2, float *cdec) {
+    const int64_t s = (idx / 16) * 16 + slot_spin((int)(idx % 16));
+    cdec[idx] = s < ns ? (float)(-1.4426950408889634 / T2[s]) : 0.0f;
+}
+
+// idx < ngroups * NCB * 64: B operand of the coil GEMM for one (group,
+// 16-column block, lane): rows k' = 8g + 2v + comp (g = lane/16) are spin
+// g + 4v of the group, real (comp 0) or imaginary (comp 1) part of the state;
+// column n = 16 cb + lane%16 is coil n/2, real (even n) or imaginary (odd n)
+// part of the signal.  csm has column stride ldc.
+__device__ inline void bop_elem(int64_t idx, int64_t ns, int64_t nc, int NCB, const double2 *csm,
+                                int64_t ldc, const double *csc, half8_t *bop) {
+    const int lane = (int)(idx % 64);
+    const int64_t t = idx / 64;
+    const int cb = (int)(t % NCB);
+    const int64_t grp = t / NCB;
+    const int n = 16 * cb + lane % 16;
+    const int64_t coil = n / 2;
+    const int part = n % 2;
+    const int g = lane / 16;
+    const double inv = coil < nc ? ldexp(1.0, kOpScale) / csc[coil] : 0.0;
+    half8_t hi, lo;
+    for (int v = 0; v < 4; ++v) {
+        const int64_t s = grp * 16 + g + 4 * v;
+        double cre = 0.0, cim = 0.0;
+        if (s < ns && coil < nc) {
+            const double2 w = csm[s + coil * ldc];
+            cre = w.x * inv;
+            cim = w.y * inv;
+        }
+        // state re multiplies (re col: cre, im col: cim); state im: (-cim, cre)
+        const double b0 = part == 0 ? cre : cim;
+        const double b1 = part == 0 ? -cim : cre;
+        _Float16 h, l;
+        split16(b0, h, l);
+        hi[2 * v] = h;
+        lo[2 * v] = l;
+        split16(b1, h, l);
+        hi[2 * v + 1] = h;
+        lo[2 * v + 1] = l;
+    }
+    bop[((grp * NCB + cb) * 2 + 0) * 64 + lane] = hi;
+    bop[((grp * NCB + cb) * 2 + 1) * 64 + lane] = lo;
+}
+
+// ---- per-segment spin-side steps ----
+
+// Block-wide max of |M0| over spins s = base + threadIdx.x (256 threads), one
+// atomic per workgroup (nonnegative floats order like their bit patterns).
+__device__ inline void m0max_block(int64_t s, int64_t ns, const double2 *mxy, float *out) {
+    __shared__ float red[4];
+    float v = 0.0f;
+    if (s < ns) {
+        const double2 m = mxy[s];
+        v = (float)sqrt(m.x * m.x + m.y * m.y);
+    }
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        if (m > 0.0f) atomicMax(reinterpret_cast<unsigned int *>(out), __float_as_uint(m));
+    }
+    __syncthreads();  // red is reused by the next call of a persistent block
+}
+
+// idx < ngroups * 16, MFMA output-row order: th0 = arg(M0) / 2pi and
+// lm = log2(|M0| / S) + kOpScale.
+__device__ inline void seg_elem(int64_t idx, int64_t ns, const double2 *mxy, float m0max,
+                                double *th0, float *lm) {
+    const int64_t s = (idx / 16) * 16 + slot_spin((int)(idx % 16));
+    double a = 0.0;
+    float l = -INFINITY;
+    if (s < ns) {
+        const double2 m = mxy[s];
+        const double mag = sqrt(m.x * m.x + m.y * m.y);
+        a = atan2(m.y, m.x) / kTwoPi;
+        l = mag > 0.0 ? (float)(log2(mag / col_scale(m0max)) + kOpScale) : -INFINITY;
+    }
+    th0[idx] = a;
+    lm[idx] = l;
+}
+
+// gi < ngroups * 16: seed[(ab * ngroups) * 16 + gi] = th0 + sum_q b_q * anc[ab][q]
+// (FP64) for ab < nab: the FP64-accumulator seed of each anchor block for the
+// spin of row slot gi % 16.  The basis is evaluated once per spin.
+__device__ inline void seed_elem(int64_t gi, int64_t ns, int64_t ngroups, int64_t nab,
+                                 const PhaseSplit &ps, const double *x, const double *y,
+                                 const double *z, const double *dw, const double *th0,
+                                 const double *anc, double *seed) {
+    const int64_t s = (gi / 16) * 16 + slot_spin((int)(gi % 16));
+    double b[12];
+    for (int q = 0; q < 12; ++q)
+        b[q] = (s < ns && q < ps.nho) ? basis_term(ps.codeho[q], x[s], y[s], z[s], dw[s]) : 0.0;
+    const double t0 = th0[gi];
+    for (int64_t ab = 0; ab < nab; ++ab) {
+        double v = t0;
+        for (int q = 0; q < ps.nho; ++q) v += b[q] * anc[ab * ps.nho + q];
+        seed[ab * ngroups * 16 + gi] = v;
+    }
+}
+
+// idx < nadc * nc, coil fastest so a wave reads contiguous partial rows; FP64
+// sum over spin tiles in tile order, scales, then the spin-independent
+// zeroth-order phase in FP64.
+__device__ inline void finalize_elem(int64_t idx, int64_t nc, int64_t nstiles, int64_t rows,
+                                     int64_t cols, const float *partials, float m0max,
+                                     const double *csc, const double *kr0, double2 *out,
+                                     int64_t ld) {
+    const int64_t c = idx % nc, k = idx / nc;
+    double re = 0.0, im = 0.0;
+    for (int64_t st = 0; st < nstiles; ++st) {
+        const float2 p = *reinterpret_cast<const float2 *>(partials + (st * rows + k) * cols + 2 * c);
+        re += p.x;
+        im += p.y;
+    }
+    const double sc = ldexp(col_scale(m0max) * csc[c], -2 * kOpScale);
+    re *= sc;
+    im *= sc;
+    const double r = kr0[k] - floor(kr0[k]);
+    double sn, cs;
+    sincos(kTwoPi * r, &sn, &cs);
+    out[k + c * ld] = double2{re * cs - im * sn, re * sn + im * cs};
+}
+
+// s < ns: FP64 end-of-segment state, Mxy <- M0 e^{-T/T2} e^{i phi_end},
+// Mz <- Mz e^{-T/T1} + rho (1 - e^{-T/T1}).
+__device__ inline void end_elem(int64_t s, const TermList &tl, const double *x, const double *y,
+                                const double *z, const double *dw, const double *rho,
+                                const double *T1, const double *T2, const double *kend, double T,
+                                double2 *mxy, double *mz) {
+    double rev = kend[tl.n];
+    for (int i = 0; i < tl.n; ++i) rev += basis_term(tl.code[i], x[s], y[s], z[s], dw[s]) * kend[i];
+    const double r = rev - floor(rev);
+    double sn, cs;
+    sincos(kTwoPi * r, &sn, &cs);
+    const double e2 = exp(-T / T2[s]);
+    const double2 m = mxy[s];
+    mxy[s] = double2{(m.x * cs - m.y * sn) * e2, (m.x * sn + m.y * cs) * e2};
+    const double e1 = exp(-T / T1[s]);
+    mz[s] = mz[s] * e1 + rho[s] * (1.0 - e1);
+}
+
+// ---- fused phase + state + coil GEMM ----
+
+// State element (d cos, d sin) of one spin-sample, split into FP16 halves:
+// hi = fp16(x) as one opaque packed register, lo = fp16(x - hi) from it.
+struct StateSplit {
+    half2_t hi, lo;  // (re, im)
+};
+
+// rf: phase in revolutions, already reduced to about [0, 1) (v_sin/v_cos take
+// revolutions and accept |rf| <= 256).
+__device__ inline StateSplit state_split(float rf, float cd, float tk, float lmv) {
+    const float sn = __builtin_amdgcn_sinf(rf);   // sin(2 pi rf)
+    const float cs = __builtin_amdgcn_cosf(rf);
+    const float d = __builtin_amdgcn_exp2f(fmaf(cd, tk, lmv));
+    const float re = d * cs, im = d * sn;
+    StateSplit o;
+    o.hi = opaque2(half2_t{(_Float16)re, (_Float16)im});
+    o.lo = half2_t{(_Float16)(re - (float)o.hi[0]), (_Float16)(im - (float)o.hi[1])};
+    return o;
+}
+
+// FP64 phase (revolutions) reduced to [0, 1) and rounded to FP32.
+__device__ inline float reduce_rev(double ph) { return (float)__builtin_amdgcn_fract(ph); }
+
+// Operands of one main-kernel pass: one segment chunk (tiles [tile_lo,
+// tile_hi) of the segment tables, sample blocks from sb_lo) and one range of
+// coil blocks [cb0, cb0 + NCB) of ncb_total.  seed holds anchor blocks from
+// ab_lo; partials hold the chunk's rows only.
+struct MainOps {
+    int64_t ngroups;
+    int gpt;
+    int cb0, ncb_total;
+    int64_t tile_lo, tile_hi, sb_lo, ab_lo;
+    const double *basis64;
+    const half8_t *aho;
+    const float *cdec;
+    const float *lm;
+    const double *seed;
+    const half8_t *bop;
+    const double *k64;
+    const half8_t *bho;
+    const float *tk;
+    float *partials;
+    const double *basisA;  // all-FP64 body
+    const double *th0;
+    const double *kr;
+};
+
+// Work item of one workgroup: spin tile and chunk-relative sample block.
+struct WorkItem {
+    int64_t stile, tile0, anchor;
+    bool valid;
+};
+
+__device__ inline WorkItem make_item(const MainOps &o, int64_t sblock, int64_t stile) {
+    const int w = threadIdx.x >> 6;
+    const int64_t sb = o.sb_lo + sblock;
+    WorkItem it;
+    it.stile = stile;
+    it.tile0 = (sb * kWaves + w) * kTiles;
+    it.anchor = sb / kBlocksPerAnchor - o.ab_lo;
+    it.valid = it.tile0 < o.tile_hi;
+    return it;
+}
+
+template <int NCB>
+__device__ inline void write_partials(const floatx4 (&acc)[kTiles][NCB], const WorkItem &it,
+                                      const MainOps &o) {
+    const int lane = threadIdx.x & 63;
+    const int64_t rows = (o.tile_hi - o.tile_lo) * 16, cols = 16 * (int64_t)o.ncb_total;
+#pragma unroll
+    for (int t = 0; t < kTiles; ++t) {
+        const int64_t tile = it.tile0 + t;
+        if (tile >= o.tile_hi) continue;
+#pragma unroll
+        for (int c = 0; c < NCB; ++c)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t row = (tile - o.tile_lo) * 16 + 4 * (lane >> 4) + v;
+                const int64_t col = 16 * (o.cb0 + c) + (lane & 15);
+                o.partials[(it.stile * rows + row) * cols + col] = acc[t][c][v];
+            }
+    }
+}
+
+// LDS layout of the split-phase body: two group records (double buffer) and
+// one private offset-operand slice per wave.
+template <int NMH, int NCB>
+struct SplitLds {
+    static constexpr int kBopChunks = NCB * 2 * 64;    // 16-byte chunks
+    static constexpr int kAhoChunks = NMH * 64;
+    static constexpr int kRecChunks = kBopChunks + kAhoChunks + 32 + 4 + 4 + 8;
+    static constexpr int kRecBytes = kRecChunks * 16;
+    static constexpr int kBvBytes = kTiles * (NMH > 0 ? NMH : 1) * 64 * 16;  // per wave
+    static constexpr int kBytes = 2 * kRecBytes + kWaves * kBvBytes;
+    static constexpr int kStage = (kRecChunks + 255) / 256;  // chunks per thread
+};
+
+// Split-phase body (FP64 first order + FP16 higher-order offsets) for one
+// work item.  Workgroup = kWaves waves over one spin tile; wave w owns kTiles
+// consecutive 16-sample tiles and accumulates a 16-sample x (16 NCB)-column
+// FP32 tile per sample tile over every 16-spin group of the spin tile.
+//
+// The spin-group operands (coil B operand, higher-order A operand, FP64
+// basis, decay rate, log-magnitude, phase seed) are the same for all waves:
+// the workgroup copies each group's record into LDS once, double-buffered,
+// while it computes the previous group (global loads into staging registers
+// before the compute, LDS stores after it, one barrier per group).  Each wave
+// keeps its own per-tile FP16 offset operands in a private LDS slice.  Every
+// wave takes part in the copies and barriers, with or without tiles; the
+// body ends on a barrier, so a persistent caller may reuse smem at once.
+template <int NMH, int NCB>
+__device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem &it,
+                                                unsigned char *smem) {
+    using L = SplitLds<NMH, NCB>;
+    // Opaque thread id: a persistent caller runs this body in a loop, and
+    // lane-dependent addresses must not be hoisted out of it (they would stay
+    // live across items and push the kernel past its register budget).
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int lane = tid & 63;
+    const int w = tid >> 6;
+    const bool active = it.valid;
+
+    uint4 *bvl = reinterpret_cast<uint4 *>(smem + 2 * L::kRecBytes + w * L::kBvBytes);
+    double kv[kTiles];
+    float tkv[kTiles];
+#pragma unroll
+    for (int t = 0; t < kTiles; ++t) {
+        const int64_t tile = it.tile0 + t;
+        const bool ok = active && tile < o.tile_hi;
+        kv[t] = ok ? o.k64[tile * 64 + lane] : 0.0;
+        tkv[t] = ok ? o.tk[tile * 16 + (lane & 15)] : 0.0f;
+#pragma unroll
+        for (int m = 0; m < NMH; ++m)
+            bvl[(t * NMH + m) * 64 + lane] =
+                ok ? reinterpret_cast<const uint4 *>(o.bho)[(tile * NMH + m) * 64 + lane] : uint4{0, 0, 0, 0};
+    }
+
+    // Each thread copies chunks tid, tid + 256, ... of every group record.
+    // Chunk c of group grp lives at src[j] + grp * stride[j] (bytes).
+    const double *seed_b = o.seed + it.anchor * o.ngroups * 16;
+    const unsigned char *src[L::kStage];
+    int64_t stride[L::kStage];
+#pragma unroll
+    for (int j = 0; j < L::kStage; ++j) {
+        int c = tid + 256 * j;
+        const unsigned char *b = reinterpret_cast<const unsigned char *>(o.bop);
+        int64_t st = 0, off = 0;
+        if (c >= L::kRecChunks) {
+            c = 0;  // unused slot: never loaded
+        }
+        if (c < L::kBopChunks) {
+            b = reinterpret_cast<const unsigned char *>(o.bop);
+            off = ((int64_t)o.cb0 * 2 * 64 + c) * 16;
+            st = (int64_t)o.ncb_total * 2 * 64 * 16;
+        } else if ((c -= L::kBopChunks) < L::kAhoChunks) {
+            b = reinterpret_cast<const unsigned char *>(o.aho);
+            off = (int64_t)c * 16;
+            st = (int64_t)NMH * 64 * 16;
+        } else if ((c -= L::kAhoChunks) < 32) {
+            b = reinterpret_cast<const unsigned char *>(o.basis64);
+            off = (int64_t)c * 16;
+            st = 64 * 8;
+        } else if ((c -= 32) < 4) {
+            b = reinterpret_cast<const unsigned char *>(o.cdec);
+            off = (int64_t)c * 16;
+            st = 16 * 4;
+        } else if ((c -= 4) < 4) {
+            b = reinterpret_cast<const unsigned char *>(o.lm);
+            off = (int64_t)c * 16;
+            st = 16 * 4;
+        } else {
+            c -= 4;
+            b = reinterpret_cast<const unsigned char *>(seed_b);
+            off = (int64_t)c * 16;
+            st = 16 * 8;
+        }
+        src[j] = b + off;
+        stride[j] = st;
+    }
+    uint4 stage[L::kStage];
+    auto load_stage = [&](int64_t grp) {
+#pragma unroll
+        for (int j = 0; j < L::kStage; ++j)
+            if (tid + 256 * j < L::kRecChunks)
+                stage[j] = *reinterpret_cast<const uint4 *>(src[j] + grp * stride[j]);
+    };
+    auto store_stage = [&](int buf) {
+        uint4 *dst = reinterpret_cast<uint4 *>(smem + buf * L::kRecBytes);
+#pragma unroll
+        for (int j = 0; j < L::kStage; ++j) {
+            const int c = tid + 256 * j;
+            if (c < L::kRecChunks) dst[c] = stage[j];
+        }
+    };
+
+    floatx4 acc[kTiles][NCB];
+#pragma unroll
+    for (int t = 0; t < kTiles; ++t)
+#pragma unroll
+        for (int c = 0; c < NCB; ++c) acc[t][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    const int64_t g0 = it.stile * o.gpt;
+    const int64_t g1 = std::min<int64_t>(g0 + o.gpt, o.ngroups);
+    load_stage(g0);
+    store_stage(0);
+    __syncthreads();
+    for (int64_t grp = g0; grp < g1; ++grp) {
+        const int cur = (int)((grp - g0) & 1);
+        const bool more = grp + 1 < g1;
+        if (more) load_stage(grp + 1);
+        if (active) {
+            const unsigned char *rec = smem + cur * L::kRecBytes;
+            const half8_t *rb = reinterpret_cast<const half8_t *>(rec);
+            half8_t bh[NCB], bl[NCB];
+#pragma unroll
+            for (int c = 0; c < NCB; ++c) {
+                bh[c] = rb[(c * 2 + 0) * 64 + lane];
+                bl[c] = rb[(c * 2 + 1) * 64 + lane];
+            }
+            half8_t ah[NMH > 0 ? NMH : 1];
+#pragma unroll
+            for (int m = 0; m < NMH; ++m) ah[m] = rb[L::kBopChunks + m * 64 + lane];
+            const unsigned char *tail = rec + (L::kBopChunks + L::kAhoChunks) * 16;
+            const double ba = reinterpret_cast<const double *>(tail)[lane];
+            const floatx4 cd = reinterpret_cast<const floatx4 *>(tail + 512)[lane >> 4];
+            const floatx4 lmv = reinterpret_cast<const floatx4 *>(tail + 576)[lane >> 4];
+            const doublex4 sd = reinterpret_cast<const doublex4 *>(tail + 640)[lane >> 4];
+            const half8_t *bv = reinterpret_cast<const half8_t *>(bvl);
+#pragma unroll
+            for (int t = 0; t < kTiles; ++t) {
+                const doublex4 ph = __builtin_amdgcn_mfma_f64_16x16x4f64(ba, kv[t], sd, 0, 0, 0);
+                floatx4 off = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int m = 0; m < NMH; ++m)
+                    off = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[m], bv[(t * NMH + m) * 64 + lane],
+                                                                 off, 0, 0, 0);
+                uint4 ahu, alu;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    float rf = reduce_rev(ph[v]);
+                    if (NMH > 0) rf = fmaf(off[v], 0x1p-20f, rf);
+                    const StateSplit e = state_split(rf, cd[v], tkv[t], lmv[v]);
+                    (&ahu.x)[v] = __builtin_bit_cast(unsigned int, e.hi);
+                    (&alu.x)[v] = __builtin_bit_cast(unsigned int, e.lo);
+                }
+                const half8_t ahi = __builtin_bit_cast(half8_t, ahu);
+                const half8_t alo = __builtin_bit_cast(half8_t, alu);
+#pragma unroll
+                for (int c = 0; c < NCB; ++c) {
+                    acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bh[c], acc[t][c], 0, 0, 0);
+                    acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bl[c], acc[t][c], 0, 0, 0);
+                    acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo, bh[c], acc[t][c], 0, 0, 0);
+                }
+            }
+#if HOM_SCHED_VALU > 0
+#pragma unroll
+            for (int i = 0; i < kTiles * (1 + NMH + 3 * NCB); ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);               // one MFMA
+                __builtin_amdgcn_sched_group_barrier(0x002, HOM_SCHED_VALU, 0);  // then VALU
+            }
+#endif
+        }
+        if (more) store_stage(cur ^ 1);
+        __syncthreads();
+    }
+    if (active) write_partials<NCB>(acc, it, o);
+}
+
+// All-FP64 phase body (NI FP64 MFMAs over every term, accumulator seeded
+// with arg(M0)), used when a segment's higher-order offsets do not fit FP16.
+// No LDS and no barriers.
+template <int NI, int NCB>
+__device__ __forceinline__ void main64_body(const MainOps &o, const WorkItem &it) {
+    if (!it.valid) return;
+    const int lane = threadIdx.x & 63;
+    double krv[kTiles][NI];
+    float tkv[kTiles];
+#pragma unroll
+    for (int t = 0; t < kTiles; ++t) {
+        const int64_t tile = it.tile0 + t;
+        const bool ok = tile < o.tile_hi;
+#pragma unroll
+        for (int i = 0; i < NI; ++i) krv[t][i] = ok ? o.kr[(tile * NI + i) * 64 + lane] : 0.0;
+        tkv[t] = ok ? o.tk[tile * 16 + (lane & 15)] : 0.0f;
+    }
+    floatx4 acc[kTiles][NCB];
+#pragma unroll
+    for (int t = 0; t < kTiles; ++t)
+#pragma unroll
+        for (int c = 0; c < NCB; ++c) acc[t][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    const int64_t g0 = it.stile * o.gpt;
+    const int64_t g1 = std::min<int64_t>(g0 + o.gpt, o.ngroups);
+    for (int64_t grp = g0; grp < g1; ++grp) {
+        double ba[NI];
+#pragma unroll
+        for (int i = 0; i < NI; ++i) ba[i] = o.basisA[(grp * NI + i) * 64 + lane];
+        const floatx4 cd = *reinterpret_cast<const floatx4 *>(o.cdec + grp * 16 + 4 * (lane >> 4));
+        const floatx4 lmv = *reinterpret_cast<const floatx4 *>(o.lm + grp * 16 + 4 * (lane >> 4));
+        const doublex4 ph0 = *reinterpret_cast<const doublex4 *>(o.th0 + grp * 16 + 4 * (lane >> 4));
+        half8_t bh[NCB], bl[NCB];
+#pragma unroll
+        for (int c = 0; c < NCB; ++c) {
+            bh[c] = o.bop[((grp * o.ncb_total + o.cb0 + c) * 2 + 0) * 64 + lane];
+            bl[c] = o.bop[((grp * o.ncb_total + o.cb0 + c) * 2 + 1) * 64 + lane];
+        }
+#pragma unroll
+        for (int t = 0; t < kTiles; ++t) {
+            doublex4 ph = ph0;
+#pragma unroll
+            for (int i = 0; i < NI; ++i)
+                ph = __builtin_amdgcn_mfma_f64_16x16x4f64(ba[i], krv[t][i], ph, 0, 0, 0);
+            uint4 ahu, alu;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const StateSplit e = state_split(reduce_rev(ph[v]), cd[v], tkv[t], lmv[v]);
+                (&ahu.x)[v] = __builtin_bit_cast(unsigned int, e.hi);
+                (&alu.x)[v] = __builtin_bit_cast(unsigned int, e.lo);
+            }
+            const half8_t ahi = __builtin_bit_cast(half8_t, ahu);
+            const half8_t alo = __builtin_bit_cast(half8_t, alu);
+#pragma unroll
+            for (int c = 0; c < NCB; ++c) {
+                acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bh[c], acc[t][c], 0, 0, 0);
+                acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bl[c], acc[t][c], 0, 0, 0);
+                acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo, bh[c], acc[t][c], 0, 0, 0);
+            }
+        }
+    }
+    write_partials<NCB>(acc, it, o);
+}
+
+inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace hom

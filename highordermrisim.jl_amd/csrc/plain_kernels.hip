@@ -328,7 +328,8 @@ hipError_t reduce_t(hom_plan *p, const void *states, int64_t lds, int64_t nadc, 
         ScopedTimer timer(p, HOM_K_COIL_PARTIALS);
         hipLaunchKernelGGL((k_coil_partials<T>), dim3((unsigned)ntiles, (unsigned)(ktiles * ctiles)),
                            dim3(256), 0, p->stream, p->ns, nadc, nc, (const Cx<T> *)states, lds,
-                           (const Cx<T> *)p->d.csm, p->ns, (Cx<T> *)p->d.partials, nch);
+                           (const Cx<T> *)p->d.csm, p->csm_ld ? p->csm_ld : p->ns,
+                           (Cx<T> *)p->d.partials, nch);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

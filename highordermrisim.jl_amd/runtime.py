@@ -37,6 +37,17 @@ ABI_SYMBOLS = (
     "hom_buffer_download", "hom_plan_set_timing", "hom_plan_kernel_time",
 )
 
+# Every symbol declared in include/hom_serve.h.
+SERVE_SYMBOLS = (
+    "hom_server_create", "hom_server_destroy", "hom_server_last_error", "hom_server_get_config",
+    "hom_server_put_phantom", "hom_server_put_csm", "hom_server_put_sequence",
+    "hom_server_release", "hom_server_submit", "hom_server_footprint", "hom_server_step",
+    "hom_server_drain", "hom_server_wait", "hom_server_status", "hom_server_result_shape",
+    "hom_server_result", "hom_server_times", "hom_server_get_stats", "hom_server_autotune",
+)
+REQUEST_STATES = {0: "queued", 1: "active", 2: "done", 3: "failed"}
+RESIDENT_KINDS = {"phantom": 0, "csm": 1, "sequence": 2}
+
 
 class HomDeviceError(RuntimeError):
     """Device, memory or state failure reported by the library."""
@@ -52,6 +63,30 @@ class SequenceHost(C.Structure):
     _fields_ = [("n_nodes", C.c_int64), ("t", _dp), ("dt", _dp), ("gx", _dp), ("gy", _dp),
                 ("gz", _dp), ("b1", _dp), ("df", _dp), ("psi", _dp), ("adc", _u8p),
                 ("excitation", _u8p), ("h", _dp * 16)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("device_bytes", C.c_int64), ("host_bytes", C.c_int64),
+                ("max_active", C.c_int), ("step_budget", C.c_int64), ("use_graphs", C.c_int),
+                ("gpt", C.c_int), ("main_grid", C.c_int), ("threaded", C.c_int)]
+
+
+class RequestDesc(C.Structure):
+    _fields_ = [("phantom", C.c_int64), ("spin_first", C.c_int64), ("spin_count", C.c_int64),
+                ("csm", C.c_int64), ("coil_count", C.c_int64), ("sequence", C.c_int64),
+                ("node_first", C.c_int64), ("node_last", C.c_int64), ("flags", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("queued", C.c_int64), ("active", C.c_int64), ("steps", C.c_int64),
+                ("graph_launches", C.c_int64), ("graphs_captured", C.c_int64),
+                ("max_active_seen", C.c_int64), ("device_bytes_used", C.c_int64),
+                ("device_bytes_peak", C.c_int64), ("busy_seconds", C.c_double),
+                ("spin_samples", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 _lib = None
@@ -94,6 +129,27 @@ def load_library():
         "hom_buffer_download": (C.c_int, [P, _vp, _vp, C.c_int64]),
         "hom_plan_set_timing": (C.c_int, [P, C.c_int]),
         "hom_plan_kernel_time": (C.c_int, [P, C.c_int, _dp, _i64p]),
+        "hom_server_create": (C.c_int, [C.POINTER(_vp), C.POINTER(ServerConfig)]),
+        "hom_server_destroy": (C.c_int, [P]),
+        "hom_server_last_error": (C.c_char_p, [P]),
+        "hom_server_get_config": (C.c_int, [P, C.POINTER(ServerConfig)]),
+        "hom_server_put_phantom": (C.c_int, [P, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i64p]),
+        "hom_server_put_csm": (C.c_int, [P, C.c_int64, C.c_int64, _dp, C.c_int64, _i64p]),
+        "hom_server_put_sequence": (C.c_int, [P, C.POINTER(SequenceHost), C.c_double, _i64p]),
+        "hom_server_release": (C.c_int, [P, C.c_int, C.c_int64]),
+        "hom_server_submit": (C.c_int, [P, C.POINTER(RequestDesc), _i64p]),
+        "hom_server_footprint": (C.c_int, [P, C.POINTER(RequestDesc), _i64p]),
+        "hom_server_step": (C.c_int, [P, C.POINTER(C.c_int)]),
+        "hom_server_drain": (C.c_int, [P]),
+        "hom_server_wait": (C.c_int, [P, C.c_int64, C.c_double, C.POINTER(C.c_int)]),
+        "hom_server_status": (C.c_int, [P, C.c_int64, C.POINTER(C.c_int)]),
+        "hom_server_result_shape": (C.c_int, [P, C.c_int64, _i64p, _i64p]),
+        "hom_server_result": (C.c_int, [P, C.c_int64, _dp, C.c_int64]),
+        "hom_server_times": (C.c_int, [P, C.c_int64, _dp, _dp, _dp]),
+        "hom_server_get_stats": (C.c_int, [P, C.POINTER(ServerStats)]),
+        "hom_server_autotune": (C.c_int, [P, C.POINTER(RequestDesc), C.POINTER(C.c_int), C.c_int,
+                                          C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
+                                          C.POINTER(C.c_int), _dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -120,6 +176,20 @@ def _raise(code: int, msg: str):
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _sequence_host(seq: HO_DiscreteSequence):
+    """SequenceHost view of a sequence plus the arrays it points into."""
+    keep = [_f64(seq.t), _f64(seq.dt if seq.dt.size else np.zeros(1)), _f64(seq.gx),
+            _f64(seq.gy), _f64(seq.gz), np.ascontiguousarray(seq.b1, np.complex128),
+            _f64(seq.df), _f64(seq.psi)]
+    adc = np.ascontiguousarray(seq.adc, np.uint8)
+    exc = np.ascontiguousarray(seq.excitation if seq.excitation.size else np.zeros(1), np.uint8)
+    h = np.ascontiguousarray(seq.h, np.float64)
+    hp = (_dp * 16)(*[C.cast(h.ctypes.data + j * h.strides[0], _dp) for j in range(16)])
+    s = SequenceHost(seq.n_nodes, *[a.ctypes.data_as(_dp) for a in keep],
+                     adc.ctypes.data_as(_u8p), exc.ctypes.data_as(_u8p), hp)
+    return s, (keep, adc, exc, h, hp)
 
 
 class DeviceBuffer:
@@ -217,15 +287,7 @@ class Plan:
         seq.validate()
         if seq.n_nodes != self.n_nodes:
             raise DimensionMismatch("The sequence node count does not match the plan.")
-        keep = [_f64(seq.t), _f64(seq.dt if seq.dt.size else np.zeros(1)), _f64(seq.gx),
-                _f64(seq.gy), _f64(seq.gz), np.ascontiguousarray(seq.b1, np.complex128),
-                _f64(seq.df), _f64(seq.psi)]
-        adc = np.ascontiguousarray(seq.adc, np.uint8)
-        exc = np.ascontiguousarray(seq.excitation if seq.excitation.size else np.zeros(1), np.uint8)
-        h = np.ascontiguousarray(seq.h, np.float64)
-        hp = (_dp * 16)(*[C.cast(h.ctypes.data + j * h.strides[0], _dp) for j in range(16)])
-        s = SequenceHost(seq.n_nodes, *[a.ctypes.data_as(_dp) for a in keep],
-                         adc.ctypes.data_as(_u8p), exc.ctypes.data_as(_u8p), hp)
+        s, _keep = _sequence_host(seq)
         self._check(self.lib.hom_upload_sequence(self.handle, C.byref(s)))
 
     def reset_state(self):
@@ -302,3 +364,138 @@ class Plan:
         n = C.c_int64(0)
         self._check(self.lib.hom_plan_kernel_time(self.handle, KERNELS[name], C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+class Server:
+    """ctypes layer over include/hom_serve.h (one method per entry point)."""
+
+    def __init__(self, device: int = 0, device_bytes: int = 0, host_bytes: int = 0,
+                 max_active: int = 0, step_budget: int = 0, use_graphs: bool = True,
+                 gpt: int = 0, main_grid: int = 0, threaded: bool = False):
+        self.lib = load_library()
+        cfg = ServerConfig(int(device), int(device_bytes), int(host_bytes), int(max_active),
+                           int(step_budget), 1 if use_graphs else 0, int(gpt), int(main_grid),
+                           1 if threaded else 0)
+        h = _vp()
+        rc = self.lib.hom_server_create(C.byref(h), C.byref(cfg))
+        if rc != HOM_OK:
+            _raise(rc, self.lib.hom_server_last_error(None).decode())
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.hom_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != HOM_OK:
+            _raise(rc, self.lib.hom_server_last_error(self.handle).decode())
+
+    def config(self) -> dict:
+        c = ServerConfig()
+        self._check(self.lib.hom_server_get_config(self.handle, C.byref(c)))
+        return {name: getattr(c, name) for name, _ in c._fields_}
+
+    def put_phantom(self, x, y, z, rho, T1, T2, dw) -> int:
+        arrs = [_f64(a) for a in (x, y, z, rho, T1, T2, dw)]
+        out = C.c_int64(0)
+        self._check(self.lib.hom_server_put_phantom(self.handle, len(arrs[0]),
+                                                    *[a.ctypes.data_as(_dp) for a in arrs],
+                                                    C.byref(out)))
+        return out.value
+
+    def put_csm(self, phantom: int, csm) -> int:
+        csm = np.asfortranarray(csm, dtype=np.complex128)
+        out = C.c_int64(0)
+        self._check(self.lib.hom_server_put_csm(self.handle, int(phantom), csm.shape[1],
+                                                csm.ctypes.data_as(_dp), max(csm.shape[0], 1),
+                                                C.byref(out)))
+        return out.value
+
+    def put_sequence(self, seq: HO_DiscreteSequence, gamma: float = GAMMA) -> int:
+        seq.validate()
+        s, _keep = _sequence_host(seq)
+        out = C.c_int64(0)
+        self._check(self.lib.hom_server_put_sequence(self.handle, C.byref(s), float(gamma),
+                                                     C.byref(out)))
+        return out.value
+
+    def release(self, kind: str, ident: int):
+        self._check(self.lib.hom_server_release(self.handle, RESIDENT_KINDS[kind], int(ident)))
+
+    @staticmethod
+    def request(phantom, spin_first, spin_count, csm, coil_count, sequence, node_first,
+                node_last, flags) -> RequestDesc:
+        return RequestDesc(int(phantom), int(spin_first), int(spin_count), int(csm),
+                           int(coil_count), int(sequence), int(node_first), int(node_last),
+                           int(flags))
+
+    def submit(self, req: RequestDesc) -> int:
+        out = C.c_int64(0)
+        self._check(self.lib.hom_server_submit(self.handle, C.byref(req), C.byref(out)))
+        return out.value
+
+    def footprint(self, req: RequestDesc) -> int:
+        out = C.c_int64(0)
+        self._check(self.lib.hom_server_footprint(self.handle, C.byref(req), C.byref(out)))
+        return out.value
+
+    def step(self) -> bool:
+        w = C.c_int(0)
+        self._check(self.lib.hom_server_step(self.handle, C.byref(w)))
+        return bool(w.value)
+
+    def drain(self):
+        self._check(self.lib.hom_server_drain(self.handle))
+
+    def wait(self, rid: int, timeout: float = -1.0) -> str:
+        st = C.c_int(0)
+        self._check(self.lib.hom_server_wait(self.handle, int(rid), float(timeout), C.byref(st)))
+        return REQUEST_STATES[st.value]
+
+    def status(self, rid: int) -> str:
+        st = C.c_int(0)
+        self._check(self.lib.hom_server_status(self.handle, int(rid), C.byref(st)))
+        return REQUEST_STATES[st.value]
+
+    def times(self, rid: int):
+        a, b, c = C.c_double(0), C.c_double(0), C.c_double(0)
+        self._check(self.lib.hom_server_times(self.handle, int(rid), C.byref(a), C.byref(b),
+                                              C.byref(c)))
+        return a.value, b.value, c.value
+
+    def result(self, rid: int) -> np.ndarray:
+        rows, cols = C.c_int64(0), C.c_int64(0)
+        self._check(self.lib.hom_server_result_shape(self.handle, int(rid), C.byref(rows),
+                                                     C.byref(cols)))
+        out = np.zeros((rows.value, cols.value), dtype=np.complex128, order="F")
+        self._check(self.lib.hom_server_result(self.handle, int(rid), out.ctypes.data_as(_dp),
+                                               max(rows.value, 1)))
+        return out
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        self._check(self.lib.hom_server_get_stats(self.handle, C.byref(st)))
+        return st.as_dict()
+
+    def autotune(self, probe: RequestDesc, gpts=None, grids=None):
+        gl = (C.c_int * len(gpts))(*gpts) if gpts else None
+        rl = (C.c_int * len(grids))(*grids) if grids else None
+        bg, bgrid, bs = C.c_int(0), C.c_int(0), C.c_double(0)
+        self._check(self.lib.hom_server_autotune(self.handle, C.byref(probe), gl,
+                                                 len(gpts) if gpts else 0, rl,
+                                                 len(grids) if grids else 0, C.byref(bg),
+                                                 C.byref(bgrid), C.byref(bs)))
+        return bg.value, bgrid.value, bs.value

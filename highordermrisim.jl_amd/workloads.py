@@ -277,3 +277,27 @@ def make_config(index: int, spin_subsample: int | None = None, gamma=GAMMA):
     if spin_subsample:
         obj = obj[::spin_subsample]
     return obj, seq, cfg
+
+
+def serving_inputs(n_spins: int = 1 << 20, n_coils: int = 32, n_samples: int = 20_000,
+                   seed: int = CONFIG_SEEDS[3], gamma=GAMMA):
+    """Resident inputs of the serving stream: a 3D phantom with config-0
+    tissue distributions on a near-cubic grid over 0.2 x 0.2 x 0.07 m (the
+    first ``n_spins`` grid points), ``n_coils`` Gaussian sensitivity maps, and
+    one 90-degree hard RF followed by an ``n_samples`` spiral readout with
+    config-0 field terms.  Requests take spin ranges, coil prefixes and
+    readout prefixes of these."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nz = max(1, int(round((n_spins / 4) ** (1 / 3))))
+    nxy = int(math.ceil(math.sqrt(n_spins / nz)))
+    xs, ys = grid_centres(nxy, 0.2), grid_centres(nxy, 0.2)
+    zs = grid_centres(nz, 0.07) if nz > 1 else np.zeros(1)
+    X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+    x, y, z = (A.ravel(order="F")[:n_spins] for A in (X, Y, Z))
+    rho, T1, T2, dw = tissue(rng, n_spins)
+    csm = (np.ones((n_spins, 1), dtype=np.complex128, order="F") if n_coils == 1
+           else gaussian_csm(x, y, z, n_coils, rng))
+    cfg = Config("serving-resident", (nxy, nxy, nz), (0.2, 0.2, 0.07), n_coils, 1, n_samples, None)
+    seq = config_sequence(cfg, rng, gamma)
+    obj = HO_Phantom(x=x, y=y, z=z, rho=rho, T1=T1, T2=T2, dw=dw, csm=csm, name=cfg.name)
+    return obj, seq

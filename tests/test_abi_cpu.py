@@ -97,3 +97,47 @@ def test_sequence_validation():
 def test_package_surface():
     for name in ("BlochHighOrder", "HO_Phantom", "HO_DiscreteSequence", "get_sim_ranges", "GAMMA"):
         assert hasattr(hom_amd, name)
+
+
+def serve_header_symbols():
+    text = (ROOT / "include" / "hom_serve.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(hom_server_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_serving_symbol():
+    lib = rt.load_library()
+    declared = serve_header_symbols()
+    assert len(declared) >= 15
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert set(declared) == set(rt.SERVE_SYMBOLS)
+
+
+def test_server_config_checks_before_device():
+    for kw, msg in [({"max_active": 65}, "max_active"), ({"gpt": 100}, "gpt"),
+                    ({"main_grid": 12}, "multiple of 8"), ({"step_budget": -1}, "step budget"),
+                    ({"device_bytes": -5}, "Arena")]:
+        with pytest.raises(ValueError, match=msg):
+            rt.Server(**kw)
+    if rt.device_count() == 0:
+        with pytest.raises(rt.HomDeviceError, match="No HIP device"):
+            rt.Server()
+
+
+def test_request_stream_distribution():
+    from hom_amd.serving import request_stream
+    a = request_stream(3, 2000, rate=200.0, n_spins_max=1 << 20)
+    b = request_stream(3, 2000, rate=200.0, n_spins_max=1 << 20)
+    assert a == b
+    arr = np.array([r.arrival for r in a])
+    assert np.all(np.diff(arr) > 0)
+    assert abs(len(a) / arr[-1] - 200.0) < 20.0  # Poisson rate
+    ns = np.array([r.spin_count for r in a])
+    assert ns.min() >= 1000 and ns.max() <= 1_000_000
+    # log-uniform: the median is near the geometric mean sqrt(1e3 * 1e6)
+    assert 0.6 < np.median(ns) / np.sqrt(1e9) < 1.6
+    assert all(0 <= r.spin_first <= (1 << 20) - r.spin_count for r in a)
+    assert {r.n_coils for r in a} == {1, 8, 16, 32}
+    assert {r.order for r in a} == {"1100", "1110", "1111"}
+    assert min(r.n_samples for r in a) >= 1000 and max(r.n_samples for r in a) <= 20000
