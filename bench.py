@@ -11,7 +11,13 @@ signal.  Prints one JSON line; progress goes to stderr.
 
 Also measured: the plain GPU-FP64 drop-in (literal per-block algorithm, 32-node
 blocks) on the same workload, signal NRMSE vs the f64 oracle on a spin
-subsample, and the oracle's CPU throughput on that subsample.
+subsample, the oracle's CPU throughput on that subsample, and the serving
+stream: Poisson arrivals (200 requests/s) of simulations with spin counts
+log-uniform in 1e3..1e6, readouts uniform in 1000..20000 samples, 1/8/16/32
+coils and masks 1100/1110/1111 over resident inputs (1,048,576-spin phantom,
+32 coil maps, RF + 20000-sample spiral), served by the continuous-batching
+server; its p50/p99 latency (arrival to signal on the host) goes in
+``latency``.
 """
 from __future__ import annotations
 
@@ -69,6 +75,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-subsample", type=int, default=128)
     ap.add_argument("--block-length", type=int, default=32)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-budget", type=int, default=0,
+                    help="spin-samples per request per step (0: server default)")
+    ap.add_argument("--serve-no-graphs", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -83,6 +96,9 @@ def main():
     if device_count() < 1:
         log("no HIP device visible")
         sys.exit(3)
+    if args.serve_only:
+        print(json.dumps({"serving": serve_benchmark(args)}), flush=True)
+        return
     t0 = time.perf_counter()
     obj, seq, cfg = make_config(args.config, spin_subsample=args.subsample if args.subsample > 1 else None)
     method = BlochHighOrder("1111")
@@ -196,6 +212,12 @@ def main():
         log(f"NRMSE vs f64 oracle: {accuracy['nrmse_vs_f64_oracle']:.3e} "
             f"(max per interleave {max(per):.3e})")
 
+    serving = None if args.no_serve else serve_benchmark(args)
+    latency = {"source": "serving stream", **{k: serving[k] for k in ("p50_ms", "p99_ms")}} \
+        if serving else {}
+    latency.update({"plan_step_p50_ms": extra["p50_step_ms"],
+                    "plan_step_p99_ms": extra["p99_step_ms"]})
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -217,9 +239,58 @@ def main():
         "plain_f64_dropin": plain,
         "speedup_vs_plain_f64": (value / plain["value"]) if plain else None,
         "accuracy": accuracy,
-        "latency": extra,
+        "latency": latency,
+        "serving": serving,
+        "kernels": {"finalize_ms_avg": extra["finalize_ms_avg"], "prep_ms_avg": extra["prep_ms_avg"]},
     }
     print(json.dumps(out), flush=True)
+
+
+def serve_benchmark(args) -> dict:
+    """Serving stream over resident inputs; returns latency and throughput."""
+    from hom_amd.serving import SimulationServer, request_stream, run_stream
+    from hom_amd.workloads import CONFIG_SEEDS, serving_inputs
+
+    t0 = time.perf_counter()
+    sobj, sseq = serving_inputs()
+    log(f"serving inputs: spins={len(sobj)} coils={sobj.n_coils} nodes={sseq.n_nodes} "
+        f"built in {time.perf_counter() - t0:.1f}s")
+    with SimulationServer(memory_bytes=64 << 30, result_bytes=4 << 30, threaded=True,
+                          step_budget=args.serve_budget or None,
+                          use_graphs=not args.serve_no_graphs) as srv:
+        ph, sq = srv.put_phantom(sobj), srv.put_sequence(sseq)
+        warm = request_stream(CONFIG_SEEDS[3] + 1, 40, rate=args.serve_rate,
+                              n_spins_max=len(sobj))
+        run_stream(srv, ph, sq, warm)
+        srv.drain()
+        before = srv.stats()
+        stream = request_stream(CONFIG_SEEDS[3], args.serve_requests, rate=args.serve_rate,
+                                n_spins_max=len(sobj))
+        log(f"serving {len(stream)} requests at {args.serve_rate:g}/s "
+            f"(offered {sum(r.spin_count * r.n_samples for r in stream) / stream[-1].arrival:.3e} "
+            "spin-samples/s)")
+        res = run_stream(srv, ph, sq, stream)
+        srv.drain()
+        after = srv.stats()
+        cfg = srv.config()
+    res.pop("requests_list")
+    res.update({
+        "rate_req_per_s": args.serve_rate,
+        "steps": after["steps"] - before["steps"],
+        "graph_launches": after["graph_launches"] - before["graph_launches"],
+        "graphs_captured": after["graphs_captured"],
+        "max_active": after["max_active_seen"],
+        "device_busy_s": after["busy_seconds"] - before["busy_seconds"],
+        "step_budget": cfg["step_budget"], "gpt": cfg["gpt"], "main_grid": cfg["main_grid"],
+        "use_graphs": bool(cfg["use_graphs"]),
+        "inputs": f"resident {len(sobj)} spins x {sobj.n_coils} coils, RF + {sseq.adc.sum()}-sample "
+                  "spiral; requests: spins log-U[1e3,1e6], samples U[1000,20000], "
+                  "coils {1,8,16,32}, masks {1100,1110,1111}",
+    })
+    log(f"serving: p50 {res['p50_ms']:.1f} ms, p99 {res['p99_ms']:.1f} ms, "
+        f"throughput {res['throughput']:.3e} spin-samples/s over {res['wall_s']:.2f}s, "
+        f"{res['steps']} steps, max active {res['max_active']}")
+    return res
 
 
 if __name__ == "__main__":

@@ -3,16 +3,17 @@
 // Device side: persistent batched versions of the mixed path's per-segment
 // kernels (mixed_common.h bodies).  Each reads one step descriptor from
 // device memory: per-request records (JobDesc), per-kernel request lists
-// with block prefix sums, and the fused-kernel work entries grouped by kernel
-// class (split phase with NMH offset MFMAs or all-FP64 phase with NI FP64
-// MFMAs, times NCB coil blocks).  Grids are fixed, so one captured hipGraph
-// per set of classes replays every step; only the descriptor changes.
+// with block prefix sums, and the fused-kernel work entries, each tagged with
+// its kernel class (split phase with NMH offset MFMAs or all-FP64 phase with
+// NI FP64 MFMAs, times NCB coil blocks).  One fused kernel serves every
+// class, switching per work item, so the step is a fixed chain of seven
+// kernels with fixed grids: one captured hipGraph replays every step and only
+// the descriptor changes.
 //
 // Fused-kernel items are spread over the eight XCDs (block b runs on XCD
-// b % 8): XCD x takes items [x T/8, (x+1) T/8) of a class from its own
-// atomic counter, so consecutive items, which share a spin tile, hit that
-// XCD's L2.  Every workgroup leaves its loop once its counter passes the end
-// of its range.
+// b % 8): XCD x takes items [x T/8, (x+1) T/8) from its own atomic counter,
+// so consecutive items, which share a spin tile, hit that XCD's L2.  Every
+// workgroup leaves its loop once its counter passes the end of its range.
 //
 // Host side: resident store (phantoms, CSMs, sequences in HBM), a first-fit
 // device arena that requests are admitted against, a pinned host arena for
@@ -23,6 +24,7 @@
 
 #include "../../include/hom_serve.h"
 
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -38,7 +40,7 @@ namespace serve {
 
 constexpr int kMaxJobs = HOM_SERVE_MAX_ACTIVE;
 constexpr int kSplitClasses = 12;  // nmh 0..2 x ncb 1..4
-constexpr int kClasses = 28;       // + all-FP64: NI 1..4 x ncb 1..4
+constexpr int kClasses = 28;       // + all-FP64: NI 1..4 x ncb 1..4 (run_class cases)
 constexpr int kMaxMain = 4 * kMaxJobs;
 constexpr int kLightGrid = 1024;
 constexpr int64_t kAlign = 256;
@@ -62,6 +64,7 @@ struct JobDesc {
     int64_t ld;
     const double *kend;
     double T;
+    int32_t do_seg, do_seed;  // segment start (th0, lm, scale) / anchor seeds this step
     PhaseSplit ps;
     TermList tl;
 };
@@ -76,14 +79,16 @@ struct ListDesc {
 struct MainDesc {
     MainOps o;
     int64_t nsb;      // sample blocks of the chunk
-    int64_t item_lo;  // first item within the class
+    int64_t item_lo;  // first item of the entry
+    int32_t cls;      // kernel class
 };
 
 struct StepDesc {
-    unsigned int ctr[kClasses][8];
-    int32_t cls_begin[kClasses], cls_end[kClasses];
-    int64_t cls_total[kClasses];
-    ListDesc m0, seg, seed, fin, end;
+    unsigned int ctr[8];       // per-XCD item counters (zero on upload)
+    float m0acc[kMaxJobs];     // per job slot: max |M0| of a starting segment (zero on upload)
+    int32_t n_main;
+    int64_t total_items;
+    ListDesc m0, ss, fin, end;  // m0max; seg + seed; finalize; end of segment
     JobDesc jobs[kMaxJobs];
     MainDesc main[kMaxMain];
 };
@@ -100,65 +105,60 @@ __device__ inline int find_entry(const ListDesc &L, int64_t b) {
     return lo;
 }
 
-__global__ void k_srv_zero(const StepDesc *__restrict__ sd) {
-    const int i = threadIdx.x;
-    if (i < sd->m0.n) *sd->jobs[sd->m0.job[i]].m0max = 0.0f;
-}
-
-__global__ __launch_bounds__(256) void k_srv_m0max(const StepDesc *__restrict__ sd) {
+// Max |M0| of every request starting a segment, into m0acc[slot].
+__global__ __launch_bounds__(256) void k_srv_m0max(const StepDesc *__restrict__ sd,
+                                                   float *__restrict__ m0acc) {
     const ListDesc &L = sd->m0;
     const int64_t total = L.prefix[L.n];
     for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
         const int e = find_entry(L, b);
         const JobDesc &J = sd->jobs[L.job[e]];
-        m0max_block((b - L.prefix[e]) * 256 + threadIdx.x, J.ns, J.mxy, J.m0max);
+        m0max_block((b - L.prefix[e]) * 256 + threadIdx.x, J.ns, J.mxy, m0acc + L.job[e]);
     }
 }
 
-__global__ __launch_bounds__(256) void k_srv_seg(const StepDesc *__restrict__ sd) {
-    const ListDesc &L = sd->seg;
+// Per spin slot: segment start (th0, lm; the scale is kept in the request's
+// memory for the later chunks) and then the chunk's anchor seeds, which read
+// the th0 the same thread just wrote.
+__global__ __launch_bounds__(256) void k_srv_segseed(const StepDesc *__restrict__ sd,
+                                                     const float *__restrict__ m0acc) {
+    const ListDesc &L = sd->ss;
     const int64_t total = L.prefix[L.n];
     for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
         const int e = find_entry(L, b);
-        const JobDesc &J = sd->jobs[L.job[e]];
-        const int64_t idx = (b - L.prefix[e]) * 256 + threadIdx.x;
-        if (idx < J.ngroups * 16) seg_elem(idx, J.ns, J.mxy, *J.m0max, J.th0, J.lm);
-    }
-}
-
-__global__ __launch_bounds__(256) void k_srv_seed(const StepDesc *__restrict__ sd) {
-    const ListDesc &L = sd->seed;
-    const int64_t total = L.prefix[L.n];
-    for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
-        const int e = find_entry(L, b);
-        const JobDesc &J = sd->jobs[L.job[e]];
+        const int slot = L.job[e];
+        const JobDesc &J = sd->jobs[slot];
         const int64_t gi = (b - L.prefix[e]) * 256 + threadIdx.x;
-        if (gi < J.ngroups * 16)
+        if (J.do_seg) {
+            const float m0 = m0acc[slot];
+            if (gi == 0) *J.m0max = m0;
+            if (gi < J.ngroups * 16) seg_elem(gi, J.ns, J.mxy, m0, J.th0, J.lm);
+        }
+        if (J.do_seed && gi < J.ngroups * 16)
             seed_elem(gi, J.ns, J.ngroups, J.nab, J.ps, J.x, J.y, J.z, J.dw, J.th0, J.anc, J.seed);
     }
 }
 
-__global__ __launch_bounds__(256) void k_srv_finalize(const StepDesc *__restrict__ sd) {
-    const ListDesc &L = sd->fin;
-    const int64_t total = L.prefix[L.n];
+// Finalize of every chunk (blocks [0, F)) and end of every finished segment
+// (blocks [F, F + E)); the two touch disjoint data.
+__global__ __launch_bounds__(256) void k_srv_finend(const StepDesc *__restrict__ sd) {
+    const ListDesc &F = sd->fin, &E = sd->end;
+    const int64_t nf = F.prefix[F.n], total = nf + E.prefix[E.n];
     for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
-        const int e = find_entry(L, b);
-        const JobDesc &J = sd->jobs[L.job[e]];
-        const int64_t idx = (b - L.prefix[e]) * 256 + threadIdx.x;
-        if (idx < J.nadc * J.nc)
-            finalize_elem(idx, J.nc, J.nstiles, J.rows, J.cols, J.partials, *J.m0max, J.csc, J.kr0,
-                          J.sig, J.ld);
-    }
-}
-
-__global__ __launch_bounds__(256) void k_srv_end(const StepDesc *__restrict__ sd) {
-    const ListDesc &L = sd->end;
-    const int64_t total = L.prefix[L.n];
-    for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
-        const int e = find_entry(L, b);
-        const JobDesc &J = sd->jobs[L.job[e]];
-        const int64_t s = (b - L.prefix[e]) * 256 + threadIdx.x;
-        if (s < J.ns) end_elem(s, J.tl, J.x, J.y, J.z, J.dw, J.rho, J.T1, J.T2, J.kend, J.T, J.mxy, J.mz);
+        if (b < nf) {
+            const int e = find_entry(F, b);
+            const JobDesc &J = sd->jobs[F.job[e]];
+            const int64_t idx = (b - F.prefix[e]) * 256 + threadIdx.x;
+            if (idx < J.nadc * J.nc)
+                finalize_elem(idx, J.nc, J.nstiles, J.rows, J.cols, J.partials, *J.m0max, J.csc,
+                              J.kr0, J.sig, J.ld);
+        } else {
+            const int e = find_entry(E, b - nf);
+            const JobDesc &J = sd->jobs[E.job[e]];
+            const int64_t s = (b - nf - E.prefix[e]) * 256 + threadIdx.x;
+            if (s < J.ns)
+                end_elem(s, J.tl, J.x, J.y, J.z, J.dw, J.rho, J.T1, J.T2, J.kend, J.T, J.mxy, J.mz);
+        }
     }
 }
 
@@ -168,21 +168,44 @@ __device__ inline int64_t uniform64(int64_t v) {
     return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
-// Claims items of class cls for this workgroup's XCD until its range is done
-// and calls body(entry, chunk sample block, spin tile) for each.
-template <typename F>
-__device__ __forceinline__ void run_items(const StepDesc *__restrict__ sd, unsigned int *ctr, int cls,
-                                          int64_t *s_item, F &&body) {
+// Body of one work item for each kernel class.
+__device__ __forceinline__ void run_class(int cls, const MainOps &o, int64_t sb, int64_t st,
+                                          unsigned char *smem) {
+    const WorkItem it = make_item(o, sb, st);
+    switch (cls) {
+#define HOM_SPLIT(C, NMH, NCB) \
+    case C: main_split_body<NMH, NCB>(o, it, smem); break;
+#define HOM_FP64(C, NI, NCB) \
+    case C: main64_body<NI, NCB>(o, it); break;
+        HOM_SPLIT(0, 0, 1) HOM_SPLIT(1, 0, 2) HOM_SPLIT(2, 0, 3) HOM_SPLIT(3, 0, 4)
+        HOM_SPLIT(4, 1, 1) HOM_SPLIT(5, 1, 2) HOM_SPLIT(6, 1, 3) HOM_SPLIT(7, 1, 4)
+        HOM_SPLIT(8, 2, 1) HOM_SPLIT(9, 2, 2) HOM_SPLIT(10, 2, 3) HOM_SPLIT(11, 2, 4)
+        HOM_FP64(12, 1, 1) HOM_FP64(13, 1, 2) HOM_FP64(14, 1, 3) HOM_FP64(15, 1, 4)
+        HOM_FP64(16, 2, 1) HOM_FP64(17, 2, 2) HOM_FP64(18, 2, 3) HOM_FP64(19, 2, 4)
+        HOM_FP64(20, 3, 1) HOM_FP64(21, 3, 2) HOM_FP64(22, 3, 3) HOM_FP64(23, 3, 4)
+        HOM_FP64(24, 4, 1) HOM_FP64(25, 4, 2) HOM_FP64(26, 4, 3) HOM_FP64(27, 4, 4)
+#undef HOM_SPLIT
+#undef HOM_FP64
+        default: break;
+    }
+}
+
+// The fused kernel of a step: claims items for this workgroup's XCD until its
+// range is done; item -> entry by binary search over the entries' item_lo.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HOM_WAVES_PER_EU, HOM_WAVES_PER_EU))) void k_srv_fused(
+    const StepDesc *__restrict__ sd, unsigned int *__restrict__ ctr) {
+    __shared__ __attribute__((aligned(32))) unsigned char smem[SplitLds<2, 4>::kBytes];
+    __shared__ int64_t s_item;
     const int xcd = blockIdx.x % 8;
-    const int64_t total = sd->cls_total[cls];
+    const int64_t total = sd->total_items;
     const int64_t lo = total * xcd / 8, hi = total * (xcd + 1) / 8;
     for (;;) {
-        if (threadIdx.x == 0) *s_item = lo + (int64_t)atomicAdd(ctr + cls * 8 + xcd, 1u);
+        if (threadIdx.x == 0) s_item = lo + (int64_t)atomicAdd(ctr + xcd, 1u);
         __syncthreads();
-        const int64_t item = uniform64(*s_item);
+        const int64_t item = uniform64(s_item);
         __syncthreads();
         if (item >= hi) break;
-        int b = sd->cls_begin[cls], e = sd->cls_end[cls];
+        int b = 0, e = sd->n_main;
         while (e - b > 1) {
             const int m = (b + e) >> 1;
             if (sd->main[m].item_lo <= item) b = m;
@@ -190,27 +213,8 @@ __device__ __forceinline__ void run_items(const StepDesc *__restrict__ sd, unsig
         }
         const MainDesc &md = sd->main[b];
         const int64_t local = item - md.item_lo;
-        body(md.o, local % md.nsb, local / md.nsb);
+        run_class(md.cls, md.o, local % md.nsb, local / md.nsb, smem);
     }
-}
-
-template <int NMH, int NCB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HOM_WAVES_PER_EU, HOM_WAVES_PER_EU))) void k_srv_main(
-    const StepDesc *__restrict__ sd, unsigned int *__restrict__ ctr, int cls) {
-    __shared__ __attribute__((aligned(32))) unsigned char smem[SplitLds<NMH, NCB>::kBytes];
-    __shared__ int64_t s_item;
-    run_items(sd, ctr, cls, &s_item, [&](const MainOps &o, int64_t sb, int64_t st) {
-        main_split_body<NMH, NCB>(o, make_item(o, sb, st), smem);
-    });
-}
-
-template <int NI, int NCB>
-__global__ __launch_bounds__(256) void k_srv_main64(const StepDesc *__restrict__ sd,
-                                                    unsigned int *__restrict__ ctr, int cls) {
-    __shared__ int64_t s_item;
-    run_items(sd, ctr, cls, &s_item, [&](const MainOps &o, int64_t sb, int64_t st) {
-        main64_body<NI, NCB>(o, make_item(o, sb, st));
-    });
 }
 
 // Request preparation (direct launches at admission).
@@ -242,27 +246,6 @@ __global__ void k_srv_prep(int64_t ns, int64_t nc, int64_t ngroups, int NCB, int
 
 int split_class(int nmh, int ncb) { return nmh * 4 + (ncb - 1); }
 int fp64_class(int NI, int ncb) { return kSplitClasses + (NI - 1) * 4 + (ncb - 1); }
-
-typedef void (*MainLaunch)(dim3, hipStream_t, const StepDesc *, unsigned int *, int);
-
-template <int NMH, int NCB>
-void launch_split(dim3 g, hipStream_t st, const StepDesc *sd, unsigned int *ctr, int cls) {
-    hipLaunchKernelGGL((k_srv_main<NMH, NCB>), g, dim3(256), 0, st, sd, ctr, cls);
-}
-template <int NI, int NCB>
-void launch_fp64(dim3 g, hipStream_t st, const StepDesc *sd, unsigned int *ctr, int cls) {
-    hipLaunchKernelGGL((k_srv_main64<NI, NCB>), g, dim3(256), 0, st, sd, ctr, cls);
-}
-
-const MainLaunch kMainLaunch[kClasses] = {
-    launch_split<0, 1>, launch_split<0, 2>, launch_split<0, 3>, launch_split<0, 4>,
-    launch_split<1, 1>, launch_split<1, 2>, launch_split<1, 3>, launch_split<1, 4>,
-    launch_split<2, 1>, launch_split<2, 2>, launch_split<2, 3>, launch_split<2, 4>,
-    launch_fp64<1, 1>,  launch_fp64<1, 2>,  launch_fp64<1, 3>,  launch_fp64<1, 4>,
-    launch_fp64<2, 1>,  launch_fp64<2, 2>,  launch_fp64<2, 3>,  launch_fp64<2, 4>,
-    launch_fp64<3, 1>,  launch_fp64<3, 2>,  launch_fp64<3, 3>,  launch_fp64<3, 4>,
-    launch_fp64<4, 1>,  launch_fp64<4, 2>,  launch_fp64<4, 3>,  launch_fp64<4, 4>,
-};
 
 // --------------------------------------------------------------- host --
 
@@ -375,6 +358,7 @@ struct Job {
     size_t seg = 0;
     int64_t sb_next = 0;
     bool finishing = false;
+    bool fetching = false;  // a hom_server_result call is copying the signal out
     double t_submit = 0.0, t_admit = 0.0, t_done = 0.0;
     hom_plan view;  // borrowed device pointers for the f64 RF blocks
 
@@ -409,6 +393,7 @@ struct hom_server {
     std::deque<Job *> queue;
     std::vector<Job *> active;
     hom_server_stats stats{};
+    std::atomic<int64_t> graph_launches{0}, graphs_captured{0};  // counted outside mu
     std::mutex step_mu;  // one step at a time
     std::thread worker;
     bool stop = false;
@@ -488,9 +473,9 @@ struct Layout {
     }
 };
 
-// Validates a request and fills everything the host can know before
-// admission: segments, tables, chunking and the device layout.  Caller holds mu.
-int plan_job(hom_server *s, const hom_request &r, Job &j) {
+// Checks a request against the resident store and pins the objects it uses
+// (their refcounts keep them alive until the request retires).  Caller holds mu.
+int resolve_job_locked(hom_server *s, const hom_request &r, Job &j) {
     auto pit = s->phantoms.find(r.phantom);
     if (pit == s->phantoms.end() || pit->second->released)
         return fail_locked(s, HOM_ERR_ARGUMENT, "Unknown phantom id.");
@@ -517,11 +502,25 @@ int plan_job(hom_server *s, const hom_request &r, Job &j) {
     j.ph = &ph;
     j.cs = &cs;
     j.sq = &sq;
+    ph.refs++;
+    cs.refs++;
+    sq.refs++;
+    return HOM_OK;
+}
+
+// Fills everything the host can know before admission: segments, tables,
+// chunking and the device layout.  Runs without mu (the resident objects are
+// pinned and their host copies never change).
+int plan_job(hom_server *s, Job &j, int gpt) {
+    const hom_request &r = j.req;
+    const ResPhantom &ph = *j.ph;
+    const ResCsm &cs = *j.cs;
+    const ResSeq &sq = *j.sq;
     j.ns = r.spin_count;
     j.nc = r.coil_count;
     j.flags = r.flags;
     j.mflags = r.flags & (HOM_HO0 | HOM_HO1 | HOM_HO2 | HOM_HO3 | HOM_DW_PRECESSION);
-    j.gpt = s->cfg.gpt;
+    j.gpt = gpt;
     j.NCB = (int)((2 * j.nc + 15) / 16);
     j.ngroups = (j.ns + 15) / 16;
     j.nstiles = std::max<int64_t>((j.ngroups + j.gpt - 1) / j.gpt, 1);
@@ -609,9 +608,9 @@ int plan_job(hom_server *s, const hom_request &r, Job &j) {
     j.dev_bytes = L.total;
     j.host_bytes = align_up(16 * j.rows * j.nc);
     if (j.dev_bytes > s->dev_arena.size)
-        return fail_locked(s, HOM_ERR_MEMORY, "The request needs more device memory than the server's arena.");
+        return fail(s, HOM_ERR_MEMORY, "The request needs more device memory than the server's arena.");
     if (j.host_bytes > s->host_arena.size)
-        return fail_locked(s, HOM_ERR_MEMORY, "The request's signal does not fit the server's host arena.");
+        return fail(s, HOM_ERR_MEMORY, "The request's signal does not fit the server's host arena.");
     return HOM_OK;
 }
 
@@ -689,10 +688,21 @@ void add_list(ListDesc &L, int slot, int64_t blocks) {
 }
 
 struct StepPlan {
-    uint32_t mask = 0;
-    std::vector<std::pair<int, MainDesc>> mains;  // (class, entry)
+    std::vector<MainDesc> mains;  // item_lo holds the entry's item count until finish_mains
     int slots = 0;
 };
+
+// Writes the fused-kernel entries with item offsets into the descriptor.
+void finish_mains(StepDesc &d, const StepPlan &sp) {
+    int64_t acc = 0;
+    for (size_t i = 0; i < sp.mains.size(); ++i) {
+        d.main[i] = sp.mains[i];
+        d.main[i].item_lo = acc;
+        acc += sp.mains[i].item_lo;
+    }
+    d.n_main = (int32_t)sp.mains.size();
+    d.total_items = acc;
+}
 
 // Fills the descriptor with one chunk of job j; returns false when the job
 // has no precession work left this step (RF-only tail or finished).
@@ -736,15 +746,14 @@ bool add_chunk(hom_server *s, Job &j, StepDesc &d, StepPlan &sp) {
     const int64_t tile_lo = sb_lo * kBlockTiles, tile_hi = std::min(sb_hi * kBlockTiles, g.tiles);
     const int64_t ab_lo = sb_lo / kBlocksPerAnchor;
     const int64_t ab_hi = (sb_hi + kBlocksPerAnchor - 1) / kBlocksPerAnchor;
-    if (sb_lo == 0) {
-        add_list(d.m0, slot, (j.ns + 255) / 256);
-        add_list(d.seg, slot, (j.ngroups * 16 + 255) / 256);
-    }
-    if (!g.fallback) {
+    J.do_seg = sb_lo == 0;
+    J.do_seed = !g.fallback;
+    if (J.do_seg) add_list(d.m0, slot, (j.ns + 255) / 256);
+    if (J.do_seed) {
         J.anc = j.at<double>(j.o_anc) + g.anc_off + ab_lo * j.ps.nho;
         J.nab = ab_hi - ab_lo;
-        add_list(d.seed, slot, (j.ngroups * 16 + 255) / 256);
     }
+    if (J.do_seg || J.do_seed) add_list(d.ss, slot, (j.ngroups * 16 + 255) / 256);
     for (int cb0 = 0; cb0 < j.NCB; cb0 += 4) {
         const int ncb = std::min(4, j.NCB - cb0);
         MainDesc m{};
@@ -772,9 +781,8 @@ bool add_chunk(hom_server *s, Job &j, StepDesc &d, StepPlan &sp) {
         o.kr = j.o_kr >= 0 ? j.at<double>(j.o_kr) + g.kr_off : nullptr;
         m.nsb = sb_hi - sb_lo;
         m.item_lo = m.nsb * j.nstiles;  // item count; turned into offsets later
-        const int cls = g.fallback ? fp64_class(j.NI, ncb) : split_class(j.ps.nmh, ncb);
-        sp.mains.push_back({cls, m});
-        sp.mask |= 1u << cls;
+        m.cls = g.fallback ? fp64_class(j.NI, ncb) : split_class(j.ps.nmh, ncb);
+        sp.mains.push_back(m);
     }
     J.nadc = std::min(tile_hi * 16, g.nadc) - tile_lo * 16;
     J.rows = (tile_hi - tile_lo) * 16;
@@ -794,45 +802,50 @@ bool add_chunk(hom_server *s, Job &j, StepDesc &d, StepPlan &sp) {
     return true;
 }
 
-// Enqueues the step kernels for class mask `mask` on the server stream.
-void enqueue_kernels(hom_server *s, uint32_t mask) {
+void enqueue_prologue(hom_server *s) {
     const StepDesc *sd = s->d_desc;
-    unsigned int *ctr = &s->d_desc->ctr[0][0];
-    hipLaunchKernelGGL(k_srv_zero, dim3(1), dim3(kMaxJobs), 0, s->stream, sd);
-    hipLaunchKernelGGL(k_srv_m0max, dim3(kLightGrid), dim3(256), 0, s->stream, sd);
-    hipLaunchKernelGGL(k_srv_seg, dim3(kLightGrid), dim3(256), 0, s->stream, sd);
-    hipLaunchKernelGGL(k_srv_seed, dim3(kLightGrid), dim3(256), 0, s->stream, sd);
-    for (int c = 0; c < kClasses; ++c)
-        if (mask & (1u << c)) kMainLaunch[c](dim3((unsigned)s->cfg.main_grid), s->stream, sd, ctr, c);
-    hipLaunchKernelGGL(k_srv_finalize, dim3(kLightGrid), dim3(256), 0, s->stream, sd);
-    hipLaunchKernelGGL(k_srv_end, dim3(kLightGrid), dim3(256), 0, s->stream, sd);
+    float *m0acc = &s->d_desc->m0acc[0];
+    hipLaunchKernelGGL(k_srv_m0max, dim3(kLightGrid), dim3(256), 0, s->stream, sd, m0acc);
+    hipLaunchKernelGGL(k_srv_segseed, dim3(kLightGrid), dim3(256), 0, s->stream, sd,
+                       (const float *)m0acc);
 }
 
-int launch_step(hom_server *s, uint32_t mask) {
+void enqueue_fused(hom_server *s, int grid) {
+    hipLaunchKernelGGL(k_srv_fused, dim3((unsigned)grid), dim3(256), 0, s->stream, s->d_desc,
+                       &s->d_desc->ctr[0]);
+}
+
+// The step's fixed chain of four kernels on the server stream.
+void enqueue_kernels(hom_server *s) {
+    enqueue_prologue(s);
+    enqueue_fused(s, s->cfg.main_grid);
+    hipLaunchKernelGGL(k_srv_finend, dim3(kLightGrid), dim3(256), 0, s->stream, s->d_desc);
+}
+
+int launch_step(hom_server *s) {
     if (!s->cfg.use_graphs) {
-        enqueue_kernels(s, mask);
+        enqueue_kernels(s);
         return hip_fail(s, hipGetLastError(), "step launch");
     }
-    auto it = s->graphs.find(mask);
+    const uint32_t key = (uint32_t)s->cfg.main_grid;
+    auto it = s->graphs.find(key);
     if (it == s->graphs.end()) {
         hipGraph_t g = nullptr;
         hipGraphExec_t ex = nullptr;
         hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed);
         if (e != hipSuccess) return hip_fail(s, e, "hipStreamBeginCapture");
-        enqueue_kernels(s, mask);
+        enqueue_kernels(s);
         e = hipStreamEndCapture(s->stream, &g);
         if (e != hipSuccess) return hip_fail(s, e, "hipStreamEndCapture");
         e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
         if (e != hipSuccess) return hip_fail(s, e, "hipGraphInstantiate");
-        it = s->graphs.emplace(mask, ex).first;
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->stats.graphs_captured++;
+        it = s->graphs.emplace(key, ex).first;
+        s->graphs_captured++;
     }
     hipError_t e = hipGraphLaunch(it->second, s->stream);
     if (e != hipSuccess) return hip_fail(s, e, "hipGraphLaunch");
-    std::lock_guard<std::mutex> lk(s->mu);
-    s->stats.graph_launches++;
+    s->graph_launches++;
     return HOM_OK;
 }
 
@@ -957,29 +970,10 @@ int step_once(hom_server *s, int *worked) {
         if (j->seg >= j->segs.size()) finishing.push_back(j);
     }
     if (!rc && (sp.slots > 0)) {
-        // Main entries grouped by class, item offsets within each class.
-        std::stable_sort(sp.mains.begin(), sp.mains.end(),
-                         [](const auto &a, const auto &b) { return a.first < b.first; });
-        for (int c = 0; c < kClasses; ++c) d.cls_begin[c] = d.cls_end[c] = 0;
-        int64_t acc = 0;
-        int prev = -1;
-        for (size_t i = 0; i < sp.mains.size(); ++i) {
-            const int c = sp.mains[i].first;
-            if (c != prev) {
-                d.cls_begin[c] = (int32_t)i;
-                acc = 0;
-                prev = c;
-            }
-            const int64_t n = sp.mains[i].second.item_lo;
-            d.main[i] = sp.mains[i].second;
-            d.main[i].item_lo = acc;
-            acc += n;
-            d.cls_end[c] = (int32_t)i + 1;
-            d.cls_total[c] = acc;
-        }
+        finish_mains(d, sp);
         const size_t bytes = offsetof(StepDesc, main) + sizeof(MainDesc) * std::max<size_t>(sp.mains.size(), 1);
         hipError_t e = hipMemcpyAsync(s->d_desc, s->h_desc, bytes, hipMemcpyHostToDevice, s->stream);
-        rc = e != hipSuccess ? hip_fail(s, e, "descriptor upload") : launch_step(s, sp.mask);
+        rc = e != hipSuccess ? hip_fail(s, e, "descriptor upload") : launch_step(s);
     }
     if (!rc)
         for (Job *j : finishing) {
@@ -1083,7 +1077,7 @@ int hom_server_create(hom_server **out, const hom_server_config *config) {
     if (c.device_bytes == 0) c.device_bytes = (int64_t)8 << 30;
     if (c.host_bytes == 0) c.host_bytes = (int64_t)1 << 30;
     if (c.max_active == 0) c.max_active = HOM_SERVE_MAX_ACTIVE;
-    if (c.step_budget == 0) c.step_budget = (int64_t)1 << 31;
+    if (c.step_budget == 0) c.step_budget = (int64_t)1 << 33;
     if (c.gpt == 0) c.gpt = 256;
     auto bad = [&](const char *m) {
         g_server_create_error = m;
@@ -1348,8 +1342,16 @@ int hom_server_footprint(hom_server *s, const hom_request *req, int64_t *bytes) 
     if (!s) return HOM_ERR_ARGUMENT;
     if (!req || !bytes) return fail(s, HOM_ERR_ARGUMENT, "A pointer is NULL.");
     Job j;
+    int gpt = 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        const int rc = resolve_job_locked(s, *req, j);
+        if (rc) return rc;
+        gpt = s->cfg.gpt;
+    }
+    const int rc = plan_job(s, j, gpt);
     std::lock_guard<std::mutex> lk(s->mu);
-    const int rc = plan_job(s, *req, j);
+    release_resident_locked(s, j);
     if (rc) return rc;
     *bytes = j.dev_bytes;
     return HOM_OK;
@@ -1360,16 +1362,24 @@ int hom_server_submit(hom_server *s, const hom_request *req, int64_t *rid) {
     if (!req || !rid) return fail(s, HOM_ERR_ARGUMENT, "A pointer is NULL.");
     const double t = now_seconds();
     auto j = std::make_unique<Job>();
+    int gpt = 0;
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->fatal) return fail_locked(s, HOM_ERR_DEVICE, "The server stopped after a device error.");
-        const int rc = plan_job(s, *req, *j);
+        const int rc = resolve_job_locked(s, *req, *j);
         if (rc) return rc;
+        gpt = s->cfg.gpt;
+    }
+    // Host tables are built without the lock, so the scheduler never waits on them.
+    const int rc = plan_job(s, *j, gpt);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (rc) {
+            release_resident_locked(s, *j);
+            return rc;
+        }
         j->t_submit = t;
         j->id = s->next_id++;
-        j->ph->refs++;
-        j->cs->refs++;
-        j->sq->refs++;
         *rid = j->id;
         s->queue.push_back(j.get());
         s->jobs[j->id] = std::move(j);
@@ -1473,22 +1483,35 @@ int hom_server_result_shape(hom_server *s, int64_t rid, int64_t *rows, int64_t *
 
 int hom_server_result(hom_server *s, int64_t rid, double *out, int64_t ld) {
     if (!s) return HOM_ERR_ARGUMENT;
+    const double2 *src = nullptr;
+    int64_t rows = 0, nc = 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->jobs.find(rid);
+        if (it == s->jobs.end() || it->second->fetching)
+            return fail_locked(s, HOM_ERR_ARGUMENT, "Unknown request id.");
+        Job &j = *it->second;
+        if (j.state == HOM_REQ_FAILED) {
+            const std::string m = "The request failed: " + j.err;
+            if (j.host_off >= 0) s->host_arena.release(j.host_off, j.host_bytes);
+            s->jobs.erase(it);
+            return fail_locked(s, HOM_ERR_DEVICE, m);
+        }
+        if (j.state != HOM_REQ_DONE) return fail_locked(s, HOM_ERR_STATE, "The request has not finished.");
+        if (j.rows > 0 && (!out || ld < j.rows))
+            return fail_locked(s, HOM_ERR_DIMENSION,
+                               "The output is NULL or its leading dimension is too small.");
+        j.fetching = true;
+        src = j.result;
+        rows = j.rows;
+        nc = j.nc;
+    }
+    // The copy runs outside the lock so the scheduler is never held up by it.
+    for (int64_t c = 0; c < nc; ++c)
+        std::memcpy(out + 2 * c * ld, src + c * rows, sizeof(double2) * (size_t)rows);
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->jobs.find(rid);
-    if (it == s->jobs.end()) return fail_locked(s, HOM_ERR_ARGUMENT, "Unknown request id.");
-    Job &j = *it->second;
-    if (j.state == HOM_REQ_FAILED) {
-        const std::string m = "The request failed: " + j.err;
-        if (j.host_off >= 0) s->host_arena.release(j.host_off, j.host_bytes);
-        s->jobs.erase(it);
-        return fail_locked(s, HOM_ERR_DEVICE, m);
-    }
-    if (j.state != HOM_REQ_DONE) return fail_locked(s, HOM_ERR_STATE, "The request has not finished.");
-    if (j.rows > 0 && (!out || ld < j.rows))
-        return fail_locked(s, HOM_ERR_DIMENSION, "The output is NULL or its leading dimension is too small.");
-    for (int64_t c = 0; c < j.nc; ++c)
-        std::memcpy(out + 2 * c * ld, j.result + c * j.rows, sizeof(double2) * (size_t)j.rows);
-    s->host_arena.release(j.host_off, j.host_bytes);
+    s->host_arena.release(it->second->host_off, it->second->host_bytes);
     s->jobs.erase(it);
     s->epoch++;
     s->cv_work.notify_one();
@@ -1510,6 +1533,8 @@ int hom_server_get_stats(hom_server *s, hom_server_stats *out) {
     if (!s || !out) return HOM_ERR_ARGUMENT;
     std::lock_guard<std::mutex> lk(s->mu);
     *out = s->stats;
+    out->graph_launches = s->graph_launches.load();
+    out->graphs_captured = s->graphs_captured.load();
     out->device_bytes_used = s->dev_arena.used;
     out->device_bytes_peak = s->dev_arena.peak;
     return HOM_OK;
@@ -1538,13 +1563,18 @@ int hom_server_autotune(hom_server *s, const hom_request *probe, const int *gpt_
         std::lock_guard<std::mutex> lk(s->mu);
         if (!s->queue.empty() || !s->active.empty())
             return fail_locked(s, HOM_ERR_STATE, "The server must be idle to autotune.");
-        const int keep = s->cfg.gpt;
-        s->cfg.gpt = *std::min_element(gpts.begin(), gpts.end());  // largest partial buffer
-        const int rc = plan_job(s, *probe, j);
-        s->cfg.gpt = keep;
+        const int rc = resolve_job_locked(s, *probe, j);
         if (rc) return rc;
-        j.dev_off = s->dev_arena.alloc(j.dev_bytes);
-        if (j.dev_off < 0) return fail_locked(s, HOM_ERR_MEMORY, "The probe does not fit the device arena.");
+    }
+    {
+        // smallest gpt: most spin tiles, so the partial buffer fits every candidate
+        const int rc = plan_job(s, j, *std::min_element(gpts.begin(), gpts.end()));
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (!rc) j.dev_off = s->dev_arena.alloc(j.dev_bytes);
+        if (rc || j.dev_off < 0) {
+            release_resident_locked(s, j);
+            return rc ? rc : fail_locked(s, HOM_ERR_MEMORY, "The probe does not fit the device arena.");
+        }
         j.dev = s->dev_base + j.dev_off;
     }
     int rc = HOM_OK;
@@ -1581,36 +1611,16 @@ int hom_server_autotune(hom_server *s, const hom_request *probe, const int *gpt_
             std::memset(&d, 0, offsetof(StepDesc, jobs));
             StepPlan sp;
             add_chunk(s, j, d, sp);
-            int64_t acc = 0;
-            int prev = -1;
-            for (size_t i = 0; i < sp.mains.size(); ++i) {
-                const int c = sp.mains[i].first;
-                if (c != prev) {
-                    d.cls_begin[c] = (int32_t)i;
-                    acc = 0;
-                    prev = c;
-                }
-                const int64_t n = sp.mains[i].second.item_lo;
-                d.main[i] = sp.mains[i].second;
-                d.main[i].item_lo = acc;
-                acc += n;
-                d.cls_end[c] = (int32_t)i + 1;
-                d.cls_total[c] = acc;
-            }
+            finish_mains(d, sp);
             double t_min = -1.0;
             for (int rep = 0; rep < 3 && !rc; ++rep) {
                 std::memset(&d.ctr, 0, sizeof(d.ctr));
                 hipError_t e = hipMemcpyAsync(s->d_desc, s->h_desc, sizeof(StepDesc), hipMemcpyHostToDevice, s->stream);
                 if (e == hipSuccess) {
-                    // prep kernels of the chunk (untimed), then the fused kernels (timed)
-                    hipLaunchKernelGGL(k_srv_zero, dim3(1), dim3(kMaxJobs), 0, s->stream, s->d_desc);
-                    hipLaunchKernelGGL(k_srv_m0max, dim3(kLightGrid), dim3(256), 0, s->stream, s->d_desc);
-                    hipLaunchKernelGGL(k_srv_seg, dim3(kLightGrid), dim3(256), 0, s->stream, s->d_desc);
-                    hipLaunchKernelGGL(k_srv_seed, dim3(kLightGrid), dim3(256), 0, s->stream, s->d_desc);
+                    // prep kernels of the chunk (untimed), then the fused kernel (timed)
+                    enqueue_prologue(s);
                     hipEventRecord(e0, s->stream);
-                    for (int c = 0; c < kClasses; ++c)
-                        if (sp.mask & (1u << c))
-                            kMainLaunch[c](dim3((unsigned)grid), s->stream, s->d_desc, &s->d_desc->ctr[0][0], c);
+                    enqueue_fused(s, grid);
                     hipEventRecord(e1, s->stream);
                     e = hipStreamSynchronize(s->stream);
                 }
@@ -1634,6 +1644,7 @@ int hom_server_autotune(hom_server *s, const hom_request *probe, const int *gpt_
     hipStreamSynchronize(s->stream);
     std::lock_guard<std::mutex> lk(s->mu);
     s->dev_arena.release(j.dev_off, j.dev_bytes);
+    release_resident_locked(s, j);
     if (rc) return rc;
     s->cfg.gpt = bg;
     s->cfg.main_grid = bgrid;

@@ -49,7 +49,7 @@ typedef struct hom_server_config {
     int64_t device_bytes;  /* device arena (admission budget); 0: 8 GiB */
     int64_t host_bytes;    /* pinned result arena; 0: 1 GiB */
     int max_active;        /* requests per step, 1..HOM_SERVE_MAX_ACTIVE; 0: 64 */
-    int64_t step_budget;   /* spin-samples per request per step; 0: 2^31 */
+    int64_t step_budget;   /* spin-samples per request per step; 0: 2^33 */
     int use_graphs;        /* 1: replay captured step graphs; 0: launch directly */
     int gpt;               /* 16-spin groups per spin tile: 64, 128, 256 or 512; 0: 256 */
     int main_grid;         /* persistent workgroups of the fused kernels; 0: 3 per CU */
