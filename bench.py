@@ -25,6 +25,7 @@ import argparse
 import importlib.util
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -54,6 +55,24 @@ def load_package():
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+TRAFFIC_PROFILE = ROOT / "profiles" / "k_mix_main-traffic-3d-1M-32coil-16x8000.md"
+
+
+def measured_traffic(spin_samples_per_launch: float, workload: str):
+    """HBM bytes per launch of the fused kernel, scaled from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE measurement of this workload (bytes per
+    spin-sample x spin-samples of this launch); None when the profile is
+    absent or was taken on another workload."""
+    try:
+        text = TRAFFIC_PROFILE.read_text()
+    except OSError:
+        return None
+    vals = dict(re.findall(r"^- (\w+): ([0-9.eE+-]+|[\w.-]+)$", text, flags=re.M))
+    if vals.get("workload") != workload or "bytes_per_spin_sample" not in vals:
+        return None
+    return float(vals["bytes_per_spin_sample"]) * spin_samples_per_launch
 
 
 def ceiling(nc, nt):
@@ -151,7 +170,7 @@ def main():
         "peak": f_alg / t_min / 1e12,
         "unit": "TFLOP/s",
         "frac": kern_rate * t_min,
-        "traffic": None,
+        "traffic": measured_traffic(ss_per_launch, cfg.name),
         "kernel": "k_mix_main",
         "kernel_ms_avg": seg_ms / max(seg_n, 1),
         "kernel_spin_samples_per_s": kern_rate,
