@@ -101,6 +101,7 @@ def main():
     ap.add_argument("--serve-budget", type=int, default=0,
                     help="spin-samples per request per step (0: server default)")
     ap.add_argument("--serve-no-graphs", action="store_true")
+    ap.add_argument("--serve-no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -278,6 +279,15 @@ def serve_benchmark(args) -> dict:
                           step_budget=args.serve_budget or None,
                           use_graphs=not args.serve_no_graphs) as srv:
         ph, sq = srv.put_phantom(sobj), srv.put_sequence(sseq)
+        tuned = None
+        if not args.serve_no_autotune:
+            cus = srv.config()["main_grid"] // 3
+            gpt, grid, secs = srv.autotune(ph, sq, spins=(0, 1 << 18), coils=32, n_samples=20000,
+                                           gpts=[128, 256, 512], grids=[2 * cus, 3 * cus, 4 * cus])
+            tuned = {"gpt": gpt, "main_grid": grid, "probe_kernel_ms": secs * 1e3,
+                     "probe": "262144 spins x 20000 samples x 32 coils, mask 1111",
+                     "candidates": "gpt {128,256,512} x grid {2,3,4} workgroups per CU"}
+            log(f"autotune: gpt {gpt}, grid {grid} ({secs * 1e3:.2f} ms on the probe)")
         warm = request_stream(CONFIG_SEEDS[3] + 1, 40, rate=args.serve_rate,
                               n_spins_max=len(sobj))
         run_stream(srv, ph, sq, warm)
@@ -302,6 +312,7 @@ def serve_benchmark(args) -> dict:
         "device_busy_s": after["busy_seconds"] - before["busy_seconds"],
         "step_budget": cfg["step_budget"], "gpt": cfg["gpt"], "main_grid": cfg["main_grid"],
         "use_graphs": bool(cfg["use_graphs"]),
+        "autotune": tuned,
         "inputs": f"resident {len(sobj)} spins x {sobj.n_coils} coils, RF + {sseq.adc.sum()}-sample "
                   "spiral; requests: spins log-U[1e3,1e6], samples U[1000,20000], "
                   "coils {1,8,16,32}, masks {1100,1110,1111}",

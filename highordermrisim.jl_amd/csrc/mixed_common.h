@@ -447,6 +447,32 @@ __device__ inline void write_partials(const floatx4 (&acc)[kTiles][NCB], const W
     }
 }
 
+// Source of one 16-byte chunk of a group record: chunk c of group grp is at
+// p + grp * stride.  Record layout: coil B operand (nbop chunks), higher-order
+// A operand (naho), FP64 basis (32), decay rate (4), log-magnitude (4), seed
+// (8).  Chunks at or beyond nrec are never loaded.
+struct StageSrc {
+    const unsigned char *p;
+    int64_t stride;
+};
+
+template <int NMH>
+__device__ inline StageSrc stage_src(int c, int nbop, int naho, int nrec, const MainOps &o,
+                                     const double *seed_b) {
+    if (c >= nrec) c = 0;
+    if (c < nbop)
+        return {reinterpret_cast<const unsigned char *>(o.bop) + ((int64_t)o.cb0 * 2 * 64 + c) * 16,
+                (int64_t)o.ncb_total * 2 * 64 * 16};
+    if ((c -= nbop) < naho)
+        return {reinterpret_cast<const unsigned char *>(o.aho) + (int64_t)c * 16, (int64_t)NMH * 64 * 16};
+    if ((c -= naho) < 32)
+        return {reinterpret_cast<const unsigned char *>(o.basis64) + (int64_t)c * 16, 64 * 8};
+    if ((c -= 32) < 4) return {reinterpret_cast<const unsigned char *>(o.cdec) + (int64_t)c * 16, 16 * 4};
+    if ((c -= 4) < 4) return {reinterpret_cast<const unsigned char *>(o.lm) + (int64_t)c * 16, 16 * 4};
+    c -= 4;
+    return {reinterpret_cast<const unsigned char *>(seed_b) + (int64_t)c * 16, 16 * 8};
+}
+
 // LDS layout of the split-phase body: two group records (double buffer) and
 // one private offset-operand slice per wave.
 template <int NMH, int NCB>
@@ -501,63 +527,28 @@ __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem
                 ok ? reinterpret_cast<const uint4 *>(o.bho)[(tile * NMH + m) * 64 + lane] : uint4{0, 0, 0, 0};
     }
 
-    // Each thread copies chunks tid, tid + 256, ... of every group record.
-    // Chunk c of group grp lives at src[j] + grp * stride[j] (bytes).
+    // Each thread copies chunks tid, tid + 256 and tid + 512 of every group
+    // record.  The three slots are separate named registers: an array indexed
+    // in a loop is placed in LDS by the compiler (before unrolling makes the
+    // index constant), which costs 48 bytes of LDS per thread.
+    static_assert(L::kStage <= 3, "a group record has at most 768 chunks");
     const double *seed_b = o.seed + it.anchor * o.ngroups * 16;
-    const unsigned char *src[L::kStage];
-    int64_t stride[L::kStage];
-#pragma unroll
-    for (int j = 0; j < L::kStage; ++j) {
-        int c = tid + 256 * j;
-        const unsigned char *b = reinterpret_cast<const unsigned char *>(o.bop);
-        int64_t st = 0, off = 0;
-        if (c >= L::kRecChunks) {
-            c = 0;  // unused slot: never loaded
-        }
-        if (c < L::kBopChunks) {
-            b = reinterpret_cast<const unsigned char *>(o.bop);
-            off = ((int64_t)o.cb0 * 2 * 64 + c) * 16;
-            st = (int64_t)o.ncb_total * 2 * 64 * 16;
-        } else if ((c -= L::kBopChunks) < L::kAhoChunks) {
-            b = reinterpret_cast<const unsigned char *>(o.aho);
-            off = (int64_t)c * 16;
-            st = (int64_t)NMH * 64 * 16;
-        } else if ((c -= L::kAhoChunks) < 32) {
-            b = reinterpret_cast<const unsigned char *>(o.basis64);
-            off = (int64_t)c * 16;
-            st = 64 * 8;
-        } else if ((c -= 32) < 4) {
-            b = reinterpret_cast<const unsigned char *>(o.cdec);
-            off = (int64_t)c * 16;
-            st = 16 * 4;
-        } else if ((c -= 4) < 4) {
-            b = reinterpret_cast<const unsigned char *>(o.lm);
-            off = (int64_t)c * 16;
-            st = 16 * 4;
-        } else {
-            c -= 4;
-            b = reinterpret_cast<const unsigned char *>(seed_b);
-            off = (int64_t)c * 16;
-            st = 16 * 8;
-        }
-        src[j] = b + off;
-        stride[j] = st;
+    const StageSrc s0 = stage_src<NMH>(tid, L::kBopChunks, L::kAhoChunks, L::kRecChunks, o, seed_b);
+    const StageSrc s1 = stage_src<NMH>(tid + 256, L::kBopChunks, L::kAhoChunks, L::kRecChunks, o, seed_b);
+    const StageSrc s2 = stage_src<NMH>(tid + 512, L::kBopChunks, L::kAhoChunks, L::kRecChunks, o, seed_b);
+    const bool u0 = tid < L::kRecChunks, u1 = tid + 256 < L::kRecChunks, u2 = tid + 512 < L::kRecChunks;
+    uint4 st0 = uint4{0, 0, 0, 0}, st1 = st0, st2 = st0;
+#define HOM_LOAD_STAGE(GRP)                                                          \
+    if (u0) st0 = *reinterpret_cast<const uint4 *>(s0.p + (GRP) * s0.stride);        \
+    if (L::kStage > 1 && u1) st1 = *reinterpret_cast<const uint4 *>(s1.p + (GRP) * s1.stride); \
+    if (L::kStage > 2 && u2) st2 = *reinterpret_cast<const uint4 *>(s2.p + (GRP) * s2.stride);
+#define HOM_STORE_STAGE(BUF)                                                         \
+    {                                                                                \
+        uint4 *dst_ = reinterpret_cast<uint4 *>(smem + (BUF) * L::kRecBytes);        \
+        if (u0) dst_[tid] = st0;                                                     \
+        if (L::kStage > 1 && u1) dst_[tid + 256] = st1;                              \
+        if (L::kStage > 2 && u2) dst_[tid + 512] = st2;                              \
     }
-    uint4 stage[L::kStage];
-    auto load_stage = [&](int64_t grp) {
-#pragma unroll
-        for (int j = 0; j < L::kStage; ++j)
-            if (tid + 256 * j < L::kRecChunks)
-                stage[j] = *reinterpret_cast<const uint4 *>(src[j] + grp * stride[j]);
-    };
-    auto store_stage = [&](int buf) {
-        uint4 *dst = reinterpret_cast<uint4 *>(smem + buf * L::kRecBytes);
-#pragma unroll
-        for (int j = 0; j < L::kStage; ++j) {
-            const int c = tid + 256 * j;
-            if (c < L::kRecChunks) dst[c] = stage[j];
-        }
-    };
 
     floatx4 acc[kTiles][NCB];
 #pragma unroll
@@ -567,13 +558,15 @@ __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem
 
     const int64_t g0 = it.stile * o.gpt;
     const int64_t g1 = std::min<int64_t>(g0 + o.gpt, o.ngroups);
-    load_stage(g0);
-    store_stage(0);
+    HOM_LOAD_STAGE(g0)
+    HOM_STORE_STAGE(0)
     __syncthreads();
     for (int64_t grp = g0; grp < g1; ++grp) {
         const int cur = (int)((grp - g0) & 1);
         const bool more = grp + 1 < g1;
-        if (more) load_stage(grp + 1);
+        if (more) {
+            HOM_LOAD_STAGE(grp + 1)
+        }
         if (active) {
             const unsigned char *rec = smem + cur * L::kRecBytes;
             const half8_t *rb = reinterpret_cast<const half8_t *>(rec);
@@ -626,9 +619,13 @@ __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem
             }
 #endif
         }
-        if (more) store_stage(cur ^ 1);
+        if (more) {
+            HOM_STORE_STAGE(cur ^ 1)
+        }
         __syncthreads();
     }
+#undef HOM_LOAD_STAGE
+#undef HOM_STORE_STAGE
     if (active) write_partials<NCB>(acc, it, o);
 }
 

@@ -24,6 +24,10 @@
 
 #include <cstring>
 
+#ifndef HOM_GPT
+#define HOM_GPT 256
+#endif
+
 namespace hom {
 
 struct MixedWorkspace {
@@ -56,7 +60,7 @@ struct MixedWorkspace {
     float *partials = nullptr;
     int64_t partial_cap = 0;
     std::vector<Segment> segs;
-    int gpt = 256;  // 16-spin groups per spin tile
+    int gpt = HOM_GPT;  // 16-spin groups per spin tile
 };
 
 TermList make_terms(int flags) {
