@@ -63,6 +63,10 @@ constexpr float kHOBMax = 512.0f;
 #ifndef HOM_SCHED_VALU
 #define HOM_SCHED_VALU 0
 #endif
+// Raise the wave's issue priority while it issues its coil MFMAs (0: off).
+#ifndef HOM_SETPRIO
+#define HOM_SETPRIO 0
+#endif
 #ifndef HOM_TILES
 #define HOM_TILES 2
 #endif
@@ -604,12 +608,18 @@ __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem
                 }
                 const half8_t ahi = __builtin_bit_cast(half8_t, ahu);
                 const half8_t alo = __builtin_bit_cast(half8_t, alu);
+#if HOM_SETPRIO
+                __builtin_amdgcn_s_setprio(1);
+#endif
 #pragma unroll
                 for (int c = 0; c < NCB; ++c) {
                     acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bh[c], acc[t][c], 0, 0, 0);
                     acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bl[c], acc[t][c], 0, 0, 0);
                     acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo, bh[c], acc[t][c], 0, 0, 0);
                 }
+#if HOM_SETPRIO
+                __builtin_amdgcn_s_setprio(0);
+#endif
             }
 #if HOM_SCHED_VALU > 0
 #pragma unroll
