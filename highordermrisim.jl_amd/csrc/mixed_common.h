@@ -63,10 +63,6 @@ constexpr float kHOBMax = 512.0f;
 #ifndef HOM_SCHED_VALU
 #define HOM_SCHED_VALU 0
 #endif
-// Raise the wave's issue priority while it issues its coil MFMAs (0: off).
-#ifndef HOM_SETPRIO
-#define HOM_SETPRIO 0
-#endif
 #ifndef HOM_TILES
 #define HOM_TILES 2
 #endif
@@ -486,7 +482,8 @@ struct SplitLds {
     static constexpr int kRecChunks = kBopChunks + kAhoChunks + 32 + 4 + 4 + 8;
     static constexpr int kRecBytes = kRecChunks * 16;
     static constexpr int kBvBytes = kTiles * (NMH > 0 ? NMH : 1) * 64 * 16;  // per wave
-    static constexpr int kBytes = 2 * kRecBytes + kWaves * kBvBytes;
+    static constexpr int kBufs = 2;  // group-record buffers (double buffer)
+    static constexpr int kBytes = kBufs * kRecBytes + kWaves * kBvBytes;
     static constexpr int kStage = (kRecChunks + 255) / 256;  // chunks per thread
 };
 
@@ -516,7 +513,7 @@ __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem
     const int w = tid >> 6;
     const bool active = it.valid;
 
-    uint4 *bvl = reinterpret_cast<uint4 *>(smem + 2 * L::kRecBytes + w * L::kBvBytes);
+    uint4 *bvl = reinterpret_cast<uint4 *>(smem + L::kBufs * L::kRecBytes + w * L::kBvBytes);
     double kv[kTiles];
     float tkv[kTiles];
 #pragma unroll
@@ -566,7 +563,8 @@ __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem
     HOM_STORE_STAGE(0)
     __syncthreads();
     for (int64_t grp = g0; grp < g1; ++grp) {
-        const int cur = (int)((grp - g0) & 1);
+        const int cur = (int)((grp - g0) % L::kBufs);
+        const int nxt = (cur + 1) % L::kBufs;
         const bool more = grp + 1 < g1;
         if (more) {
             HOM_LOAD_STAGE(grp + 1)
@@ -608,18 +606,12 @@ __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem
                 }
                 const half8_t ahi = __builtin_bit_cast(half8_t, ahu);
                 const half8_t alo = __builtin_bit_cast(half8_t, alu);
-#if HOM_SETPRIO
-                __builtin_amdgcn_s_setprio(1);
-#endif
 #pragma unroll
                 for (int c = 0; c < NCB; ++c) {
                     acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bh[c], acc[t][c], 0, 0, 0);
                     acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bl[c], acc[t][c], 0, 0, 0);
                     acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo, bh[c], acc[t][c], 0, 0, 0);
                 }
-#if HOM_SETPRIO
-                __builtin_amdgcn_s_setprio(0);
-#endif
             }
 #if HOM_SCHED_VALU > 0
 #pragma unroll
@@ -630,7 +622,7 @@ __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem
 #endif
         }
         if (more) {
-            HOM_STORE_STAGE(cur ^ 1)
+            HOM_STORE_STAGE(nxt)
         }
         __syncthreads();
     }
