@@ -10,7 +10,8 @@ Layers:
 - ``runtime``    ctypes binding of the C ABI in ``libhom.so`` (include/hom_abi.h);
 - ``simulate``   the reference's ``simulate`` / ``prepare_simulation`` surface
                  on top of the ABI;
-- ``serving``    continuous-batching request loop over the ABI's server.
+- ``serving``    continuous-batching request loop over the ABI's server;
+- ``trajectory`` ``get_kspace`` (native ``hom_kspace``), ``grad2traj``, ``traj2grad``.
 """
 from .types import (GAMMA, BlochHighOrder, DimensionMismatch, HO_DiscreteSequence,  # noqa: F401
                     HO_Phantom, get_sim_ranges)
@@ -26,5 +27,6 @@ __all__ += ["Mag", "PreparedSimulation", "default_sim_params", "prepare_simulati
             "setup_simulation", "simulate"]
 
 from . import serving  # noqa: E402,F401
+from .trajectory import get_kspace, grad2traj, rf_flip_angles, traj2grad  # noqa: E402,F401
 
-__all__ += ["serving"]
+__all__ += ["serving", "get_kspace", "grad2traj", "rf_flip_angles", "traj2grad"]

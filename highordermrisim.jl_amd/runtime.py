@@ -34,7 +34,8 @@ ABI_SYMBOLS = (
     "hom_set_state", "hom_get_state", "hom_precession_block", "hom_excitation_block",
     "hom_reduce_signal", "hom_run", "hom_synchronize", "hom_signal_rows", "hom_download_signal",
     "hom_signal_element_size", "hom_buffer_alloc", "hom_buffer_free", "hom_buffer_upload",
-    "hom_buffer_download", "hom_plan_set_timing", "hom_plan_kernel_time",
+    "hom_buffer_download", "hom_plan_set_timing", "hom_plan_kernel_time", "hom_kspace",
+    "hom_kspace_rf_events",
 )
 
 # Every symbol declared in include/hom_serve.h.
@@ -129,6 +130,10 @@ def load_library():
         "hom_buffer_download": (C.c_int, [P, _vp, _vp, C.c_int64]),
         "hom_plan_set_timing": (C.c_int, [P, C.c_int]),
         "hom_plan_kernel_time": (C.c_int, [P, C.c_int, _dp, _i64p]),
+        "hom_kspace": (C.c_int, [C.POINTER(SequenceHost), C.c_double, _u8p, C.c_int64, _dp, _dp,
+                                 _dp, _dp]),
+        "hom_kspace_rf_events": (C.c_int, [C.POINTER(SequenceHost), C.c_double, _i64p, _dp,
+                                           C.c_int64]),
         "hom_server_create": (C.c_int, [C.POINTER(_vp), C.POINTER(ServerConfig)]),
         "hom_server_destroy": (C.c_int, [P]),
         "hom_server_last_error": (C.c_char_p, [P]),
@@ -499,3 +504,46 @@ class Server:
                                                  len(grids) if grids else 0, C.byref(bg),
                                                  C.byref(bgrid), C.byref(bs)))
         return bg.value, bgrid.value, bs.value
+
+
+def kspace(seq: HO_DiscreteSequence, gamma: float = GAMMA, skip_rf=None):
+    """hom_kspace: (K_nominal, K_nominal_adc, K_dfc, K_dfc_adc) of a discretized
+    sequence (host computation in the library; no device needed)."""
+    lib = load_library()
+    seq.validate()
+    if seq.n_nodes < 2:
+        raise ValueError("Cannot integrate an empty sequence.")
+    s, _keep = _sequence_host(seq)
+    n_ev = C.c_int64(0)
+    rc = lib.hom_kspace_rf_events(C.byref(s), float(gamma), C.byref(n_ev), None, 0)
+    if rc != HOM_OK:
+        _raise(rc, "invalid sequence")
+    skip = None
+    if skip_rf is not None:
+        skip = np.ascontiguousarray(np.asarray(skip_rf, dtype=bool), dtype=np.uint8)
+        if skip.size != n_ev.value:
+            raise DimensionMismatch("skip_rf must contain one flag per RF event.")
+    ni, nadc = seq.n_nodes - 1, int(np.count_nonzero(seq.adc))
+    kn = np.zeros((ni, 3), order="F")
+    kna = np.zeros((nadc, 3), order="F")
+    kd = np.zeros((ni, 16), order="F")
+    kda = np.zeros((nadc, 16), order="F")
+    rc = lib.hom_kspace(C.byref(s), float(gamma),
+                        skip.ctypes.data_as(_u8p) if skip is not None else None,
+                        skip.size if skip is not None else 0, kn.ctypes.data_as(_dp),
+                        kna.ctypes.data_as(_dp), kd.ctypes.data_as(_dp), kda.ctypes.data_as(_dp))
+    if rc != HOM_OK:
+        _raise(rc, "hom_kspace failed")
+    return kn, kna, kd, kda
+
+
+def rf_flip_angles(seq: HO_DiscreteSequence, gamma: float = GAMMA) -> np.ndarray:
+    """Flip angle (degrees) of every RF event of a discretized sequence."""
+    lib = load_library()
+    s, _keep = _sequence_host(seq)
+    n_ev = C.c_int64(0)
+    _ok = lib.hom_kspace_rf_events(C.byref(s), float(gamma), C.byref(n_ev), None, 0)
+    out = np.zeros(max(n_ev.value, 1))
+    lib.hom_kspace_rf_events(C.byref(s), float(gamma), C.byref(n_ev), out.ctypes.data_as(_dp),
+                             out.size)
+    return out[:n_ev.value]

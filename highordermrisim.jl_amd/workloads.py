@@ -86,8 +86,11 @@ def build_sequence(blocks, fields=None, dt_max=1e-3, dt_rf=5e-5, measured_first_
     ``fields(t)`` returns a 16 x len(t) array of coefficient waveforms at
     absolute times t.  When ``measured_first_order`` is true, rows 1..3 get the
     nominal gradient added (so h1..h3 equal the applied gradient plus whatever
-    ``fields`` puts there).  Adjacent blocks share their boundary node; that
-    node carries the values of the block that starts there."""
+    ``fields`` puts there).  Adjacent blocks share their boundary node when
+    their values agree there; where a waveform jumps, both sides are kept at
+    the same time with a zero-length interval between them, as the
+    reference's discretization keeps field discontinuities
+    (src/datatypes/HO_DiscreteSequence.jl:85-120)."""
     ts, gx, gy, gz, b1, df, psi, adc, exc = [], [], [], [], [], [], [], [], []
     t0 = 0.0
     for bi, blk in enumerate(blocks):
@@ -121,9 +124,14 @@ def build_sequence(blocks, fields=None, dt_max=1e-3, dt_rf=5e-5, measured_first_
             e = np.zeros(rel.size - 1, dtype=bool)
         start = 0 if bi == 0 else 1
         if bi > 0:
-            # The shared boundary node takes the new block's values.
-            gx[-1][-1], gy[-1][-1], gz[-1][-1] = g[0][0], g[1][0], g[2][0]
-            b1[-1][-1], df[-1][-1], psi[-1][-1] = b[0], d[0], p[0]
+            same = (gx[-1][-1] == g[0][0] and gy[-1][-1] == g[1][0] and gz[-1][-1] == g[2][0]
+                    and b1[-1][-1] == b[0] and df[-1][-1] == d[0] and psi[-1][-1] == p[0])
+            if same:
+                start = 1  # one shared boundary node
+            else:
+                # A jump keeps both sides at the same time: a zero-length interval.
+                start = 0
+                exc.append(np.zeros(1, dtype=bool))
         ts.append(t0 + rel[start:])
         gx.append(g[0][start:].copy()); gy.append(g[1][start:].copy()); gz.append(g[2][start:].copy())
         b1.append(b[start:].copy()); df.append(d[start:].copy()); psi.append(p[start:].copy())

@@ -160,6 +160,22 @@ int hom_buffer_download(hom_plan *plan, void *host, const void *dev, int64_t byt
 int hom_plan_set_timing(hom_plan *plan, int enable);
 int hom_plan_kernel_time(hom_plan *plan, int kernel_id, double *total_ms, int64_t *launches);
 
+/* Encoding coefficients of a discretized sequence: get_kspace / grad2kspace
+ * (reference src/datatypes/HO_Sequence.jl:221-287) on the node grid, host
+ * only (no plan, no device).  Outputs are column-major with n_nodes - 1 rows
+ * (row i = end of interval i) or one row per ADC node: k_nominal (3 columns,
+ * Gx Gy Gz, cycles/m), k_dfc (16 columns, h0..h15, cycles/m^order), and their
+ * ADC rows; any output may be NULL.  skip_rf (one flag per RF event, or NULL)
+ * continues the integral through that event instead of resetting
+ * (excitation, flip <= 90.01 degrees) or negating it (refocusing).  Needs at
+ * least two nodes (ArgumentError otherwise, as the reference). */
+int hom_kspace(const hom_sequence_host *seq, double gamma, const uint8_t *skip_rf, int64_t n_skip,
+               double *k_nominal, double *k_nominal_adc, double *k_dfc, double *k_dfc_adc);
+/* RF events (maximal runs of excitation intervals) and their flip angles in
+ * degrees (first `capacity` written), as classified by hom_kspace. */
+int hom_kspace_rf_events(const hom_sequence_host *seq, double gamma, int64_t *n_events,
+                         double *flip_deg, int64_t capacity);
+
 #ifdef __cplusplus
 }
 #endif
