@@ -25,6 +25,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from .raw import signal_to_raw_data
 from .runtime import COIL_MODES, Plan, device_count
 from .types import GAMMA, BlochHighOrder, DimensionMismatch, HO_DiscreteSequence, HO_Phantom, \
     get_sim_ranges
@@ -196,7 +197,8 @@ def _finalize(sig, state, seconds, setup: SimulationSetup):
                        "Nblocks": len(setup.parts[0]), "sim_time_sec": seconds,
                        "max_adc_per_block": setup.max_adc, "max_rf_adc_per_block": setup.max_rf_adc,
                        "adc_state_buffer_samples": setup.max_adc})
-    return {"signal": mat, "params": raw_params}
+    return signal_to_raw_data(mat, setup.seqd, phantom_name=getattr(setup.obj, "name", ""),
+                              sim_params=raw_params, gamma=float(params.get("gamma", GAMMA)))
 
 
 def _make_plan(setup: SimulationSetup) -> Plan:

@@ -7,6 +7,7 @@ import pytest
 
 import hom_amd
 import oracle
+from hom_amd.raw import RawAcquisitionData
 from hom_amd import BlochHighOrder, HO_Phantom, prepare_simulation, simulate
 from hom_amd.types import GAMMA, get_sim_ranges
 from hom_amd.workloads import FreeBlock, RFBlock, build_sequence
@@ -65,7 +66,13 @@ def test_legacy_api_state_ownership_and_kats():
     state = simulate(obj, seq, sim_params={**p, "return_type": "state"})
     np.testing.assert_allclose(state.xy, 1j * density, atol=1e-12)
     raw = simulate(obj, seq, sim_params={**p, "return_type": "raw"})
-    assert raw["signal"].shape == (5, 2, 1) and raw["params"]["Nblocks"] >= 2
+    # reference test_koma_compatibility.jl:99-105
+    assert isinstance(raw, RawAcquisitionData)
+    assert sum(pr.data.shape[0] for pr in raw.profiles) == 5
+    assert all(pr.data.shape[1] == 2 for pr in raw.profiles)
+    assert raw.params["userParameters"]["Nblocks"] >= 2
+    np.testing.assert_allclose(np.vstack([pr.data for pr in raw.profiles]),
+                               sig[:, :, 0].astype(np.complex64), rtol=1e-6, atol=1e-7)
 
 
 def test_zeroth_order_phase_and_rf_ignores_fields():
