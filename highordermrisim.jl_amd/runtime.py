@@ -35,7 +35,7 @@ ABI_SYMBOLS = (
     "hom_reduce_signal", "hom_run", "hom_synchronize", "hom_signal_rows", "hom_download_signal",
     "hom_signal_element_size", "hom_buffer_alloc", "hom_buffer_free", "hom_buffer_upload",
     "hom_buffer_download", "hom_plan_set_timing", "hom_plan_kernel_time", "hom_kspace",
-    "hom_kspace_rf_events",
+    "hom_kspace_rf_events", "hom_discretize", "hom_discrete_copy", "hom_discrete_free",
 )
 
 # Every symbol declared in include/hom_serve.h.
@@ -64,6 +64,17 @@ class SequenceHost(C.Structure):
     _fields_ = [("n_nodes", C.c_int64), ("t", _dp), ("dt", _dp), ("gx", _dp), ("gy", _dp),
                 ("gz", _dp), ("b1", _dp), ("df", _dp), ("psi", _dp), ("adc", _u8p),
                 ("excitation", _u8p), ("h", _dp * 16)]
+
+
+class Wave(C.Structure):
+    _fields_ = [("n", C.c_int64), ("t", _dp), ("a", _dp)]
+
+
+class BlockDesc(C.Structure):
+    _fields_ = [("duration", C.c_double), ("gx", Wave), ("gy", Wave), ("gz", Wave),
+                ("rf_re", Wave), ("rf_im", Wave), ("rf_t0", C.c_double), ("rf_t1", C.c_double),
+                ("rf_df", C.c_double), ("rf_phase", C.c_double), ("adc_n", C.c_int64),
+                ("adc_t", _dp), ("h", Wave * 16)]
 
 
 class ServerConfig(C.Structure):
@@ -134,6 +145,10 @@ def load_library():
                                  _dp, _dp]),
         "hom_kspace_rf_events": (C.c_int, [C.POINTER(SequenceHost), C.c_double, _i64p, _dp,
                                            C.c_int64]),
+        "hom_discretize": (C.c_int, [C.POINTER(BlockDesc), C.c_int64, C.c_double, C.c_double,
+                                     C.c_int, C.POINTER(_vp), _i64p]),
+        "hom_discrete_copy": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _u8p, _u8p, _dp]),
+        "hom_discrete_free": (C.c_int, [_vp]),
         "hom_server_create": (C.c_int, [C.POINTER(_vp), C.POINTER(ServerConfig)]),
         "hom_server_destroy": (C.c_int, [P]),
         "hom_server_last_error": (C.c_char_p, [P]),
@@ -547,3 +562,34 @@ def rf_flip_angles(seq: HO_DiscreteSequence, gamma: float = GAMMA) -> np.ndarray
     lib.hom_kspace_rf_events(C.byref(s), float(gamma), C.byref(n_ev), out.ctypes.data_as(_dp),
                              out.size)
     return out[:n_ev.value]
+
+
+def discretize_blocks(descs, keep, dt_max: float, dt_rf: float, nominal_in_h: bool):
+    """hom_discretize over a ctypes BlockDesc array; returns the node arrays."""
+    lib = load_library()
+    arr = (BlockDesc * max(len(descs), 1))(*descs)
+    n = C.c_int64(0)
+    handle = _vp()
+    rc = lib.hom_discretize(arr, len(descs), float(dt_max), float(dt_rf), int(nominal_in_h),
+                            C.byref(handle), C.byref(n))
+    if rc != HOM_OK:
+        _raise(rc, "invalid sequence block description")
+    try:
+        nn = n.value
+        t, gx, gy, gz, df, psi = (np.zeros(nn) for _ in range(6))
+        b1 = np.zeros(nn, np.complex128)
+        adc = np.zeros(nn, np.uint8)
+        exc = np.zeros(max(nn - 1, 1), np.uint8)
+        h = np.zeros((16, nn))
+        rc = lib.hom_discrete_copy(handle, t.ctypes.data_as(_dp), gx.ctypes.data_as(_dp),
+                                   gy.ctypes.data_as(_dp), gz.ctypes.data_as(_dp),
+                                   b1.ctypes.data_as(_dp), df.ctypes.data_as(_dp),
+                                   psi.ctypes.data_as(_dp), adc.ctypes.data_as(_u8p),
+                                   exc.ctypes.data_as(_u8p), h.ctypes.data_as(_dp))
+        if rc != HOM_OK:
+            _raise(rc, "hom_discrete_copy failed")
+    finally:
+        lib.hom_discrete_free(handle)
+    return HO_DiscreteSequence(t=t, gx=gx, gy=gy, gz=gz, b1=b1, df=df, psi=psi,
+                               adc=adc.astype(bool), excitation=exc[:max(nn - 1, 0)].astype(bool),
+                               h=h)

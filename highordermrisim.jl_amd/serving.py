@@ -130,6 +130,14 @@ class SimulationServer:
         qid = self.srv.put_sequence(seq, gamma)
         return ResidentSequence(qid, seq.n_nodes, np.flatnonzero(np.asarray(seq.adc, bool)))
 
+    def put_sequence_blocks(self, blocks, gamma: float = GAMMA, dt_max: float = 1e-3,
+                            dt_rf: float = 5e-5,
+                            measured_first_order: bool = True) -> ResidentSequence:
+        """Discretizes a block description (``sequence.discretize``, native)
+        and keeps the result resident."""
+        from .sequence import discretize
+        return self.put_sequence(discretize(blocks, dt_max, dt_rf, measured_first_order), gamma)
+
     def release(self, resident):
         if isinstance(resident, ResidentPhantom):
             self.srv.release("csm", resident.csm_id)

@@ -176,6 +176,43 @@ int hom_kspace(const hom_sequence_host *seq, double gamma, const uint8_t *skip_r
 int hom_kspace_rf_events(const hom_sequence_host *seq, double gamma, int64_t *n_events,
                          double *flip_deg, int64_t capacity);
 
+/* Sequence discretization (reference src/datatypes/HO_DiscreteSequence.jl:93-179,
+ * with KomaMRIBase's sampling replaced by this library's rules; host only).
+ * A waveform is piecewise linear through its knots (times relative to the
+ * block start, nondecreasing; a repeated time is a jump) and zero outside
+ * them.  B1 is nonzero only inside the RF window [rf_t0, rf_t1). */
+typedef struct hom_wave {
+    int64_t n;
+    const double *t, *a;
+} hom_wave;
+
+typedef struct hom_block {
+    double duration;                  /* s */
+    hom_wave gx, gy, gz;              /* nominal gradients, T/m */
+    hom_wave rf_re, rf_im;            /* B1, T */
+    double rf_t0, rf_t1;              /* RF window (rf_t1 <= rf_t0: no RF) */
+    double rf_df, rf_phase;           /* Hz, rad, inside the RF window */
+    int64_t adc_n;
+    const double *adc_t;              /* ADC sample times in (0, duration] */
+    hom_wave h[16];                   /* field coefficients h0..h15 */
+} hom_block;
+
+/* Samples every waveform on one node grid: block edges, knots, ADC times and
+ * RF window edges, filled to steps <= dt_max (<= dt_rf inside RF); a value
+ * jump keeps both sides at the same time (zero-length interval); blocks share
+ * their boundary node when every value agrees.  With nominal_in_h the nominal
+ * gradients are added to h1..h3 (measured first-order fields).  The result
+ * is held in *out (n_nodes nodes) until hom_discrete_free. */
+typedef struct hom_discrete hom_discrete;
+int hom_discretize(const hom_block *blocks, int64_t n_blocks, double dt_max, double dt_rf,
+                   int nominal_in_h, hom_discrete **out, int64_t *n_nodes);
+/* Copies the node arrays (any may be NULL): n values each, b1 interleaved
+ * complex (2 n), excitation n - 1, h 16 rows of n (row j contiguous). */
+int hom_discrete_copy(const hom_discrete *d, double *t, double *gx, double *gy, double *gz,
+                      double *b1, double *df, double *psi, uint8_t *adc, uint8_t *excitation,
+                      double *h);
+int hom_discrete_free(hom_discrete *d);
+
 #ifdef __cplusplus
 }
 #endif
