@@ -202,6 +202,19 @@ def main():
         plain = {"value": spin_samples / dt, "unit": "spin-samples/s", "block_length": L,
                  "seconds": dt}
         log(f"plain f64: {plain['value']:.4e} spin-samples/s ({dt:.2f}s)")
+        with Plan("f32", ns, nc, seq.n_nodes, max_adc) as pp:
+            pp.upload_phantom(obj)
+            pp.upload_sequence(seq)
+            pp.reset_state()
+            pp.run([arr[:2] for arr in parts], method)
+            pp.reset_state()
+            pp.synchronize()
+            a = time.perf_counter()
+            pp.run(parts, method)
+            pp.synchronize()
+            dt32 = time.perf_counter() - a
+        plain["f32_value"] = spin_samples / dt32
+        log(f"plain f32: {plain['f32_value']:.4e} spin-samples/s ({dt32:.2f}s)")
 
     # ---------------- accuracy + CPU baseline on a spin subsample -----------------
     accuracy, cpu = None, None

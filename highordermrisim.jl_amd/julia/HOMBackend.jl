@@ -15,7 +15,8 @@
 # Complex{Float64}); BitVector flags are widened to UInt8 first.
 module HOMBackend
 
-export HOMWorkspace, hom_simulate_shard, hom_available
+export HOMWorkspace, hom_simulate_shard, hom_available, hom_get_kspace, HOMServer,
+       hom_put_phantom!, hom_put_sequence!, hom_submit!, hom_result!
 
 const libhom = get(ENV, "HOM_LIBRARY",
     joinpath(@__DIR__, "..", "libhom.so"))
@@ -215,6 +216,154 @@ function hom_simulate_shard(obj, hoseqd, params, plan; callbacks=(), gamma, M)
     finally
         close!(ws)
     end
+end
+
+# Host view of a discretized sequence for the duration of `f(host)`.
+function with_sequence_host(f, hoseqd)
+    s = hoseqd.seqd
+    arrays = (f64vec(s.t), f64vec(s.Δt), f64vec(s.Gx), f64vec(s.Gy), f64vec(s.Gz),
+              Vector{ComplexF64}(s.B1), f64vec(s.Δf), f64vec(s.ψ))
+    adc, exc = u8vec(s.ADC), u8vec(s.excitation_bool)
+    fields = ntuple(j -> f64vec(getfield(hoseqd, j + 1)), 16)
+    GC.@preserve arrays adc exc fields begin
+        host = HOMSequenceHost(length(s.t), pointer(arrays[1]), pointer(arrays[2]),
+            pointer(arrays[3]), pointer(arrays[4]), pointer(arrays[5]),
+            Ptr{Float64}(pointer(arrays[6])), pointer(arrays[7]), pointer(arrays[8]),
+            pointer(adc), pointer(exc), ntuple(j -> pointer(fields[j]), 16))
+        return f(host)
+    end
+end
+
+"""
+    hom_get_kspace(hoseqd; gamma, skip_rf=nothing)
+
+`get_kspace` on an already discretized sequence through `hom_kspace`
+(reference src/datatypes/HO_Sequence.jl:221-287): returns
+`(K_nominal, K_nominal_adc, K_dfc, K_dfc_adc)`.
+"""
+function hom_get_kspace(hoseqd; gamma, skip_rf=nothing)
+    n = length(hoseqd.seqd.t)
+    nadc = count(hoseqd.seqd.ADC)
+    kn, kna = zeros(n - 1, 3), zeros(nadc, 3)
+    kd, kda = zeros(n - 1, 16), zeros(nadc, 16)
+    skip = skip_rf === nothing ? UInt8[] : UInt8.(collect(skip_rf))
+    st = with_sequence_host(hoseqd) do host
+        ccall((:hom_kspace, libhom), Cint,
+              (Ref{HOMSequenceHost}, Float64, Ptr{UInt8}, Int64, Ptr{Float64}, Ptr{Float64},
+               Ptr{Float64}, Ptr{Float64}),
+              host, gamma, skip_rf === nothing ? C_NULL : pointer(skip), length(skip),
+              kn, kna, kd, kda)
+    end
+    st == Cint(2) && throw(DimensionMismatch("skip_rf must contain one flag per RF event."))
+    st == HOM_OK || throw(ArgumentError("Cannot integrate an empty sequence."))
+    return kn, kna, kd, kda
+end
+
+# ---- continuous-batching server (include/hom_serve.h) ----
+
+struct HOMServerConfig
+    device::Cint
+    device_bytes::Int64
+    host_bytes::Int64
+    max_active::Cint
+    step_budget::Int64
+    use_graphs::Cint
+    gpt::Cint
+    main_grid::Cint
+    threaded::Cint
+end
+
+struct HOMRequest
+    phantom::Int64
+    spin_first::Int64
+    spin_count::Int64
+    csm::Int64
+    coil_count::Int64
+    sequence::Int64
+    node_first::Int64
+    node_last::Int64
+    flags::Cint
+end
+
+mutable struct HOMServer
+    handle::Ptr{Cvoid}
+end
+
+server_error(h::Ptr{Cvoid}) =
+    unsafe_string(ccall((:hom_server_last_error, libhom), Cstring, (Ptr{Cvoid},), h))
+
+function server_check(status::Cint, h::Ptr{Cvoid})
+    status == HOM_OK && return nothing
+    msg = server_error(h)
+    status == Cint(1) && throw(ArgumentError(msg))
+    status == Cint(2) && throw(DimensionMismatch(msg))
+    error(msg)
+end
+
+function HOMServer(; device=0, device_bytes=8 << 30, host_bytes=1 << 30, max_active=64,
+                   step_budget=0, use_graphs=true, threaded=true)
+    cfg = HOMServerConfig(device, device_bytes, host_bytes, max_active, step_budget,
+                          use_graphs, 0, 0, threaded)
+    ref = Ref{Ptr{Cvoid}}(C_NULL)
+    server_check(ccall((:hom_server_create, libhom), Cint,
+                       (Ref{Ptr{Cvoid}}, Ref{HOMServerConfig}), ref, cfg), C_NULL)
+    srv = HOMServer(ref[])
+    finalizer(srv) do s
+        s.handle == C_NULL || ccall((:hom_server_destroy, libhom), Cint, (Ptr{Cvoid},), s.handle)
+        s.handle = C_NULL
+    end
+    return srv
+end
+
+"Uploads an HO_Phantom and its CSM; returns (phantom id, csm id)."
+function hom_put_phantom!(srv::HOMServer, obj)
+    id, cid = Ref{Int64}(0), Ref{Int64}(0)
+    a = map(f64vec, (obj.x, obj.y, obj.z, obj.ρ, obj.T1, obj.T2, obj.Δw))
+    server_check(ccall((:hom_server_put_phantom, libhom), Cint,
+        (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+         Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ref{Int64}),
+        srv.handle, length(a[1]), a..., id), srv.handle)
+    csm = Matrix{ComplexF64}(obj.csm)
+    server_check(ccall((:hom_server_put_csm, libhom), Cint,
+        (Ptr{Cvoid}, Int64, Int64, Ptr{ComplexF64}, Int64, Ref{Int64}),
+        srv.handle, id[], size(csm, 2), csm, size(csm, 1), cid), srv.handle)
+    return id[], cid[]
+end
+
+function hom_put_sequence!(srv::HOMServer, hoseqd; gamma)
+    id = Ref{Int64}(0)
+    st = with_sequence_host(hoseqd) do host
+        ccall((:hom_server_put_sequence, libhom), Cint,
+              (Ptr{Cvoid}, Ref{HOMSequenceHost}, Float64, Ref{Int64}), srv.handle, host, gamma, id)
+    end
+    server_check(st, srv.handle)
+    return id[]
+end
+
+"Queues a request; returns its id (`flags` as for hom_run)."
+function hom_submit!(srv::HOMServer, phantom, csm, sequence; spins, coils, nodes, flags)
+    req = HOMRequest(phantom, first(spins) - 1, length(spins), csm, coils, sequence,
+                     first(nodes) - 1, last(nodes) - 1, flags)
+    rid = Ref{Int64}(0)
+    server_check(ccall((:hom_server_submit, libhom), Cint,
+                       (Ptr{Cvoid}, Ref{HOMRequest}, Ref{Int64}), srv.handle, req, rid), srv.handle)
+    return rid[]
+end
+
+"Waits for a request and returns its (ΣADC, Nc) signal."
+function hom_result!(srv::HOMServer, rid)
+    state = Ref{Cint}(0)
+    server_check(ccall((:hom_server_wait, libhom), Cint, (Ptr{Cvoid}, Int64, Float64, Ref{Cint}),
+                       srv.handle, rid, -1.0, state), srv.handle)
+    rows, cols = Ref{Int64}(0), Ref{Int64}(0)
+    server_check(ccall((:hom_server_result_shape, libhom), Cint,
+                       (Ptr{Cvoid}, Int64, Ref{Int64}, Ref{Int64}), srv.handle, rid, rows, cols),
+                 srv.handle)
+    sig = zeros(ComplexF64, rows[], cols[])
+    server_check(ccall((:hom_server_result, libhom), Cint,
+                       (Ptr{Cvoid}, Int64, Ptr{ComplexF64}, Int64), srv.handle, rid, sig,
+                       max(rows[], 1)), srv.handle)
+    return sig
 end
 
 end # module

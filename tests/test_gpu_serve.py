@@ -266,3 +266,33 @@ def test_serve_errors(inputs):
         srv.release(ph)
         with pytest.raises(ValueError):
             srv.submit(ph, sq, m, spins=(0, 10))
+
+
+def test_serve_block_description_sequence():
+    """A request whose sequence arrives as block descriptions (native
+    discretization) against the f64 oracle on the same discretized sequence."""
+    require_gpu()
+    from hom_amd import SeqBlock, discretize
+    obj, _ = workloads.serving_inputs(n_spins=5000, n_coils=8, n_samples=400, seed=9)
+    tau, dur = 1e-4, 4e-3
+    adc = np.arange(1, 1201) * 2e-6
+    t_knots = np.linspace(0, dur, 41)
+    blocks = [SeqBlock.hard_rf(tau, 1.0 / (4 * GAMMA * tau)),
+              SeqBlock(dur, gx=([0, 2e-4, 2.2e-3, 2.4e-3], [0, 8e-3, 8e-3, 0]),
+                       gy=(t_knots, 3e-3 * np.sin(2 * np.pi * t_knots / dur)), adc=adc,
+                       fields={0: (t_knots, 1e-6 * np.cos(t_knots * 900)),
+                               6: (t_knots, 2e-4 * np.sin(t_knots * 700)),
+                               12: (t_knots, 1e-3 * np.cos(t_knots * 300))})]
+    seqd = discretize(blocks)
+    method = BlochHighOrder("1111")
+    ref, _, _ = oracle.run_workload(obj, seqd, method.flags, GAMMA)
+    with SimulationServer(memory_bytes=1 << 30, result_bytes=64 * MB, threaded=False) as srv:
+        ph = srv.put_phantom(obj)
+        sq = srv.put_sequence_blocks(blocks)
+        print_case("serve-blocks", sq.n_nodes)
+        req = srv.submit(ph, sq, method)
+        srv.drain()
+        sig = req.result()
+    assert sig.shape == ref.shape == (1200, 8)
+    e = nrmse(sig, ref)
+    assert e <= NRMSE_BAR, f"NRMSE {e:.3e}"
