@@ -546,6 +546,13 @@ int hom_plan_set_timing(hom_plan *p, int enable) {
     return HOM_OK;
 }
 
+int hom_plan_mixed_stats(const hom_plan *p, int64_t *segments, int64_t *fp64_segments) {
+    if (!p || !segments || !fp64_segments) return HOM_ERR_ARGUMENT;
+    *segments = p->mixed_segments;
+    *fp64_segments = p->mixed_fp64_segments;
+    return HOM_OK;
+}
+
 int hom_plan_kernel_time(hom_plan *p, int id, double *total_ms, int64_t *launches) {
     if (!p || id < 0 || id >= HOM_K_COUNT || !total_ms || !launches) return HOM_ERR_ARGUMENT;
     hipSetDevice(p->device);

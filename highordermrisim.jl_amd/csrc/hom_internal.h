@@ -88,6 +88,9 @@ struct hom_plan {
     int64_t signal_rows = 0, signal_capacity_rows = 0;
 
     hom::MixedWorkspace *mixed = nullptr;
+    // Last mixed run: precession segments with samples, and how many of them
+    // ran the all-FP64 phase body.
+    int64_t mixed_segments = 0, mixed_fp64_segments = 0;
 
     bool timing = false;
     hom::TimerSlot timers[HOM_K_COUNT];

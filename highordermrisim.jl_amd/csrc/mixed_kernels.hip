@@ -418,6 +418,7 @@ int mixed_run(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *
               int flags) {
     if (!p->mixed) p->mixed = new MixedWorkspace();
     MixedWorkspace &W = *p->mixed;
+    p->mixed_segments = p->mixed_fp64_segments = 0;
     const int64_t ns = p->ns, nc = p->nc;
     int rc = HOM_OK;
     const int mflags = flags & (HOM_HO0 | HOM_HO1 | HOM_HO2 | HOM_HO3 | HOM_DW_PRECESSION);
@@ -562,6 +563,8 @@ int mixed_run(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *
             continue;
         }
         if (s.nadc > 0 && ns > 0) {
+            ++p->mixed_segments;
+            if (s.fallback) ++p->mixed_fp64_segments;
             {
                 ScopedTimer timer(p, HOM_K_MIXED_PREP);
                 hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, p->stream, W.m0max, (int64_t)1, 0.0f);

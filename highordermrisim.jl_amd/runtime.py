@@ -36,6 +36,7 @@ ABI_SYMBOLS = (
     "hom_signal_element_size", "hom_buffer_alloc", "hom_buffer_free", "hom_buffer_upload",
     "hom_buffer_download", "hom_plan_set_timing", "hom_plan_kernel_time", "hom_kspace",
     "hom_kspace_rf_events", "hom_discretize", "hom_discrete_copy", "hom_discrete_free",
+    "hom_plan_mixed_stats",
 )
 
 # Every symbol declared in include/hom_serve.h.
@@ -141,6 +142,7 @@ def load_library():
         "hom_buffer_download": (C.c_int, [P, _vp, _vp, C.c_int64]),
         "hom_plan_set_timing": (C.c_int, [P, C.c_int]),
         "hom_plan_kernel_time": (C.c_int, [P, C.c_int, _dp, _i64p]),
+        "hom_plan_mixed_stats": (C.c_int, [P, _i64p, _i64p]),
         "hom_kspace": (C.c_int, [C.POINTER(SequenceHost), C.c_double, _u8p, C.c_int64, _dp, _dp,
                                  _dp, _dp]),
         "hom_kspace_rf_events": (C.c_int, [C.POINTER(SequenceHost), C.c_double, _i64p, _dp,
@@ -384,6 +386,14 @@ class Plan:
         n = C.c_int64(0)
         self._check(self.lib.hom_plan_kernel_time(self.handle, KERNELS[name], C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def mixed_stats(self):
+        """(segments, fp64_segments) of the last mixed run: precession segments
+        that acquired samples, and those on the all-FP64 phase body."""
+        seg = C.c_int64(0)
+        f64 = C.c_int64(0)
+        self._check(self.lib.hom_plan_mixed_stats(self.handle, C.byref(seg), C.byref(f64)))
+        return seg.value, f64.value
 
 
 class Server:

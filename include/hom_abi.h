@@ -160,6 +160,13 @@ int hom_buffer_download(hom_plan *plan, void *host, const void *dev, int64_t byt
 int hom_plan_set_timing(hom_plan *plan, int enable);
 int hom_plan_kernel_time(hom_plan *plan, int kernel_id, double *total_ms, int64_t *launches);
 
+/* Diagnostics of the last hom_run with precision "mixed" (no counterpart in
+ * the reference): the number of free-precession segments that acquired
+ * samples, and how many of them ran the all-FP64 phase body because their
+ * higher-order phase offsets within a 256-sample anchor block exceed half a
+ * revolution (slower, same accuracy). */
+int hom_plan_mixed_stats(const hom_plan *plan, int64_t *segments, int64_t *fp64_segments);
+
 /* Encoding coefficients of a discretized sequence: get_kspace / grad2kspace
  * (reference src/datatypes/HO_Sequence.jl:221-287) on the node grid, host
  * only (no plan, no device).  Outputs are column-major with n_nodes - 1 rows

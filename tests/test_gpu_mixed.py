@@ -10,7 +10,8 @@ import pytest
 import oracle
 from hom_amd.runtime import Plan
 from hom_amd.types import GAMMA, BlochHighOrder, HO_Phantom, get_sim_ranges
-from hom_amd.workloads import high_order_workload, make_config
+from hom_amd.serving import SimulationServer
+from hom_amd.workloads import FreeBlock, RFBlock, build_sequence, high_order_workload, make_config
 from helpers import assert_isapprox, nrmse, print_case, require_gpu
 
 pytestmark = pytest.mark.gpu
@@ -117,6 +118,63 @@ def test_mixed_column_scaling_extreme_magnitudes():
     for c in range(3):
         assert nrmse(sig[:, c], ref[:, c]) <= NRMSE_BAR, c
     assert np.all(sig[:, 3] == 0)
+
+
+def strong_field_sequence():
+    """Two 300-sample readouts, each after a 90-degree hard pulse.  Third-order
+    fields of 1 T/m^3 during the first readout give the edge spins about 18
+    revolutions of higher-order phase inside one 256-sample anchor block, far
+    beyond what the FP16 offsets hold, so that segment runs the all-FP64
+    phase body; the second readout has 1e-3 T/m^3 and stays on the split
+    path."""
+    adc = np.linspace(1e-3 / 300, 1e-3, 300)
+
+    def grad(rel):
+        return np.vstack([np.full(rel.size, 2e-3), 1e-3 * np.sin(2e3 * rel), np.zeros(rel.size)])
+
+    def fields(t):
+        h = np.zeros((16, t.size))
+        amp = np.where(t < 1.15e-3, 1.0, 1e-3)
+        for j in range(9, 16):
+            h[j] = amp * (0.5 + 0.5 * np.sin(2e3 * t + j))
+        h[4:9] = 1e-4 * np.cos(3e3 * t)[None, :]
+        return h
+
+    pulse = RFBlock(1e-4, 90.0 / 360.0 / (GAMMA * 1e-4))
+    blocks = [pulse, FreeBlock(1e-3, adc, grad), pulse, FreeBlock(1e-3, adc, grad)]
+    return build_sequence(blocks, fields=fields, dt_max=5e-6)
+
+
+def test_mixed_fp64_fallback_segment():
+    """The all-FP64 phase body (and the split body in the same run) against
+    the f64 oracle, through a plan and through the server."""
+    require_gpu()
+    obj, _, _, _ = high_order_workload(nspins=700, ncoils=8)
+    seq = strong_field_sequence()
+    m = BlochHighOrder("1111")
+    parts = get_sim_ranges(seq, 32, 32)
+    ref, rmxy, rmz = oracle.run(obj, seq, parts, m.flags, GAMMA)
+    print_case("mixed-fp64-fallback", len(obj), seq.n_nodes)
+    with Plan("mixed", len(obj), obj.n_coils, seq.n_nodes, 64) as plan:
+        plan.upload_phantom(obj)
+        plan.upload_sequence(seq)
+        plan.run(parts, m)
+        sig = plan.download_signal()
+        mxy, mz = plan.get_state()
+        assert plan.mixed_stats() == (2, 1)
+    assert sig.shape == ref.shape == (600, 8)
+    for half in (slice(0, 300), slice(300, 600)):
+        assert nrmse(sig[half], ref[half]) <= NRMSE_BAR
+    assert_isapprox(mxy, rmxy, 1e-11, 1e-11, "Mxy")
+    assert_isapprox(mz, rmz, 1e-11, 1e-11, "Mz")
+    print_case("serve-fp64-fallback")
+    with SimulationServer(memory_bytes=1 << 28, result_bytes=1 << 24, threaded=False) as srv:
+        req = srv.submit(srv.put_phantom(obj), srv.put_sequence(seq), m)
+        srv.drain()
+        out = req.result()
+    assert out.shape == ref.shape
+    for half in (slice(0, 300), slice(300, 600)):
+        assert nrmse(out[half], ref[half]) <= NRMSE_BAR
 
 
 def test_mixed_deterministic_and_matches_f64_plan():
