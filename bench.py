@@ -179,6 +179,7 @@ def main():
         "kernel_ms_avg": seg_ms / max(seg_n, 1),
         "kernel_spin_samples_per_s": kern_rate,
         "flops_per_spin_sample": f_alg,
+        "fp32_equiv_frac": kern_rate * f_alg / P_FP32_VEC,
         "ceiling_model": {
             "spin_samples_per_s": 1.0 / t_min,
             "frac": kern_rate * t_min,
@@ -190,6 +191,33 @@ def main():
              "p50_step_ms": float(np.percentile(step_ms, 50)),
              "p99_step_ms": float(np.percentile(step_ms, 99))}
     plan.close()
+
+    # ---------------- end to end: host inputs in, host signal out -----------------
+    # A fresh plan per call: phantom/CSM and sequence uploads (H2D), the host
+    # integration of the sequence tables, the run and the signal download (D2H).
+    with Plan("mixed", ns, nc, seq.n_nodes, max_adc) as pe:
+        pe.synchronize()
+        a = time.perf_counter()
+        pe.upload_phantom(obj)
+        pe.synchronize()
+        a1 = time.perf_counter()
+        pe.upload_sequence(seq)
+        pe.synchronize()
+        a2 = time.perf_counter()
+        pe.run(parts, method)
+        pe.synchronize()
+        b = time.perf_counter()
+        sig_host = pe.download_signal()
+        c = time.perf_counter()
+    end_to_end = {"value": spin_samples / (c - a), "unit": "spin-samples/s", "seconds": c - a,
+                  "phantom_upload_ms": (a1 - a) * 1e3, "sequence_upload_ms": (a2 - a1) * 1e3,
+                  "first_run_ms": (b - a2) * 1e3, "download_ms": (c - b) * 1e3,
+                  "signal_bytes": int(sig_host.nbytes),
+                  "includes": "uploads (H2D), host table integration, run, signal download (D2H)"}
+    del sig_host
+    log(f"end to end: {end_to_end['value']:.4e} spin-samples/s ({c - a:.3f}s: phantom "
+        f"{end_to_end['phantom_upload_ms']:.1f} ms, sequence {end_to_end['sequence_upload_ms']:.1f} ms, "
+        f"first run {end_to_end['first_run_ms']:.1f} ms, download {end_to_end['download_ms']:.1f} ms)")
 
     # ---------------- plain GPU-FP64 drop-in on the same workload -----------------
     plain = None
@@ -241,6 +269,14 @@ def main():
                "sample": f"every {args.cpu_subsample}th spin ({len(sub)} spins) x all {total_adc} "
                          f"samples, {nc} coils, f64, {threads} threads, {cdt:.1f}s"}
         log(f"cpu oracle: {cpu['value']:.4e} spin-samples/s")
+        sub1 = obj[::args.cpu_subsample * 8]
+        a = time.perf_counter()
+        oracle.run(sub1, seq, parts, method.flags, GAMMA, n_threads=1)
+        c1 = time.perf_counter() - a
+        cpu["single_thread"] = {"value": len(sub1) * total_adc / c1, "unit": "spin-samples/s",
+                                "sample": f"every {args.cpu_subsample * 8}th spin ({len(sub1)} "
+                                          f"spins) x all {total_adc} samples, 1 thread, {c1:.1f}s"}
+        log(f"cpu oracle, 1 thread: {cpu['single_thread']['value']:.4e} spin-samples/s")
         with Plan("mixed", len(sub), nc, seq.n_nodes, max_adc) as ps:
             ps.upload_phantom(sub)
             ps.upload_sequence(seq)
@@ -277,6 +313,7 @@ def main():
                    "interleaves": cfg.n_interleaves, "samples_per_interleave": cfg.n_adc,
                    "mask": method.name, "precision": "mixed", "block_length": L},
         "roofline": roofline,
+        "end_to_end": end_to_end,
         "cpu_baseline": cpu,
         "plain_f64_dropin": plain,
         "speedup_vs_plain_f64": (value / plain["value"]) if plain else None,

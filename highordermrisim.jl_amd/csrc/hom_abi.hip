@@ -43,6 +43,12 @@ int ensure_alloc(hom_plan *p, void **ptr, size_t bytes) {
     return HOM_OK;
 }
 
+// Device words for the upload statistics: max(17, Nc).
+int ensure_stats(hom_plan *p) {
+    return ensure_alloc(p, (void **)&p->d.stats,
+                        sizeof(unsigned long long) * (size_t)std::max<int64_t>(17, p->nc));
+}
+
 int ensure_partials(hom_plan *p, int64_t n_adc) {
     const int64_t need = coil_partial_elements(p->ns, n_adc, p->nc);
     if (need <= p->d.partial_capacity) return HOM_OK;
@@ -232,7 +238,7 @@ int hom_plan_destroy(hom_plan *p) {
     void **all[] = {&p->d.x,  &p->d.y,  &p->d.z,  &p->d.rho, &p->d.T1, &p->d.T2,  &p->d.dw,
                     &p->d.csm, &p->d.coef, &p->d.gx, &p->d.gy, &p->d.gz, &p->d.dt, &p->d.b1,
                     &p->d.df, &p->d.psi, &p->d.mxy, &p->d.mz, &p->d.states, &p->d.partials,
-                    &p->signal};
+                    (void **)&p->d.stats, &p->signal};
     for (void **q : all) free_ptr(*q);
     if (p->d.adc) hipFree(p->d.adc);
     for (void *b : p->user_buffers) hipFree(b);
@@ -261,10 +267,19 @@ int hom_upload_phantom(hom_plan *p, const double *x, const double *y, const doub
     rc = rc ? rc : upload_real(p, &p->d.T2, T2, p->ns);
     rc = rc ? rc : upload_real(p, &p->d.dw, dw, p->ns);
     if (rc) return rc;
-    for (int j = 0; j <= 16; ++j) p->h_basis_max[j] = 0.0;
-    for (int64_t s = 0; s < p->ns; ++s)
-        for (int j = 1; j <= 16; ++j)
-            p->h_basis_max[j] = std::max(p->h_basis_max[j], std::fabs(basis_term(j, x[s], y[s], z[s], dw[s])));
+    // max |basis term j| over the spins, reduced on the device
+    unsigned long long bits[17] = {};
+    if (p->ns > 0) {
+        rc = ensure_stats(p);
+        rc = rc ? rc : check_hip(p, hipMemsetAsync(p->d.stats, 0, sizeof(bits), p->stream), "hipMemsetAsync");
+        rc = rc ? rc : check_hip(p, launch_basis_absmax(p, p->d.stats), "basis statistics");
+        rc = rc ? rc : check_hip(p, hipMemcpyAsync(bits, p->d.stats, sizeof(bits), hipMemcpyDeviceToHost,
+                                                   p->stream), "hipMemcpyAsync");
+        rc = rc ? rc : check_hip(p, hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+        if (rc) return rc;
+    }
+    p->h_basis_max[0] = 0.0;
+    for (int j = 1; j <= 16; ++j) std::memcpy(&p->h_basis_max[j], &bits[j], sizeof(double));
     p->has_phantom = true;
     if (p->mixed) mixed_free(p);  // cached operands depend on the phantom
     return hom_reset_state(p);
@@ -289,19 +304,28 @@ int hom_upload_csm(hom_plan *p, const double *csm, int64_t ld) {
                     const double *v = csm + 2 * (s + c * ld);
                     tmp[2 * (s + c * p->ns)] = (float)v[0];
                     tmp[2 * (s + c * p->ns) + 1] = (float)v[1];
-                    p->h_csm_colmax[c] = std::max(p->h_csm_colmax[c], std::hypot(v[0], v[1]));
                 }
             rc = check_hip(p, hipMemcpy(p->d.csm, tmp.data(), 4 * 2 * n, hipMemcpyHostToDevice), "hipMemcpy");
         } else {
-            std::vector<double> tmp(2 * n);
-            for (int64_t c = 0; c < p->nc; ++c)
-                for (int64_t s = 0; s < p->ns; ++s) {
-                    const double *v = csm + 2 * (s + c * ld);
-                    tmp[2 * (s + c * p->ns)] = v[0];
-                    tmp[2 * (s + c * p->ns) + 1] = v[1];
-                    p->h_csm_colmax[c] = std::max(p->h_csm_colmax[c], std::hypot(v[0], v[1]));
-                }
-            rc = check_hip(p, hipMemcpy(p->d.csm, tmp.data(), 8 * 2 * n, hipMemcpyHostToDevice), "hipMemcpy");
+            // straight from the caller's columns (column stride ld) into stride ns
+            rc = check_hip(p, hipMemcpy2D(p->d.csm, 16 * (size_t)p->ns, csm, 16 * (size_t)ld, 16 * (size_t)p->ns,
+                                          (size_t)p->nc, hipMemcpyHostToDevice),
+                           "hipMemcpy2D");
+        }
+        // column maxima of |csm| for the mixed path's power-of-two scales
+        std::vector<unsigned long long> bits((size_t)p->nc, 0ull);
+        rc = rc ? rc : ensure_stats(p);
+        rc = rc ? rc : check_hip(p, hipMemsetAsync(p->d.stats, 0, sizeof(unsigned long long) * bits.size(),
+                                                   p->stream), "hipMemsetAsync");
+        rc = rc ? rc : check_hip(p, launch_csm_colmax2(p, p->d.stats), "CSM statistics");
+        rc = rc ? rc : check_hip(p, hipMemcpyAsync(bits.data(), p->d.stats, sizeof(unsigned long long) * bits.size(),
+                                                   hipMemcpyDeviceToHost, p->stream), "hipMemcpyAsync");
+        rc = rc ? rc : check_hip(p, hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+        if (rc) return rc;
+        for (int64_t c = 0; c < p->nc; ++c) {
+            double m2;
+            std::memcpy(&m2, &bits[(size_t)c], sizeof(double));
+            p->h_csm_colmax[(size_t)c] = std::sqrt(m2);
         }
     }
     if (rc) return rc;

@@ -53,6 +53,7 @@ struct DeviceArrays {
     void *states = nullptr;               // Ns x max_adc complex (column stride Ns)
     void *partials = nullptr;             // coil reduction partials
     int64_t partial_capacity = 0;         // in complex elements
+    unsigned long long *stats = nullptr;  // upload statistics (max(17, Nc) words)
 };
 
 struct MixedWorkspace;  // mixed_kernels.hip
@@ -127,6 +128,10 @@ hipError_t launch_coil_reduce(hom_plan *p, const void *states, int64_t ld_states
 int64_t coil_partial_elements(int64_t ns, int64_t n_adc, int64_t nc);
 hipError_t launch_fill_complex(hom_plan *p, void *out, int64_t rows, int64_t cols, int64_t ld);
 hipError_t launch_reset_state(hom_plan *p);
+// Upload statistics as bit patterns of nonnegative doubles (out zeroed by the
+// caller): out[j] = max |basis term j| (j = 1..16); out[c] = max |csm[., c]|^2.
+hipError_t launch_basis_absmax(hom_plan *p, unsigned long long *out);
+hipError_t launch_csm_colmax2(hom_plan *p, unsigned long long *out);
 
 // ---- mixed path (mixed_kernels.hip) ----
 int mixed_run(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *rf, int64_t np,

@@ -276,9 +276,57 @@ int dev_upload(hom_plan *p, T **dst, const std::vector<T> &src) {
                      "hipMemcpy");
 }
 
+// Nearest-even double -> binary16 bits (inf on overflow, signed zero on
+// underflow), inline integer arithmetic in place of the runtime's conversion
+// call in the per-sample table loop.
+inline uint16_t half_bits_rne(double v) {
+    uint64_t b;
+    std::memcpy(&b, &v, 8);
+    const uint16_t sign = (uint16_t)((b >> 48) & 0x8000u);
+    const int e = (int)((b >> 52) & 0x7ff);
+    const uint64_t frac = b & 0xFFFFFFFFFFFFFull;
+    if (e == 0x7ff) return (uint16_t)(sign | 0x7c00u | (frac ? 0x200u : 0u));
+    if (e == 0) return sign;  // double subnormals lie far below half's range
+    const int E = e - 1023;
+    if (E > 15) return (uint16_t)(sign | 0x7c00u);
+    int Eh = E < -14 ? -14 : E;
+    const int shift = 42 + (Eh - E);  // drops the bits below the half ulp 2^(Eh-10)
+    if (shift >= 64) return sign;
+    const uint64_t sig = frac | (1ull << 52);
+    uint64_t q = sig >> shift;
+    const uint64_t rem = sig & ((1ull << shift) - 1), halfway = 1ull << (shift - 1);
+    if (rem > halfway || (rem == halfway && (q & 1))) ++q;
+    if (q >= 2048) {
+        q >>= 1;
+        ++Eh;
+    }
+    if (q < 1024) return (uint16_t)(sign | q);  // subnormal
+    if (Eh > 15) return (uint16_t)(sign | 0x7c00u);
+    return (uint16_t)(sign | ((Eh + 15) << 10) | (q - 1024));
+}
+
+// Exact binary16 bits -> double.
+inline double half_bits_to_double(uint16_t h) {
+    const int ex = (h >> 10) & 31, m = h & 1023;
+    double v;
+    if (ex == 0) {
+        v = (double)m * 0x1p-24;
+    } else if (ex == 31) {
+        v = m ? std::nan("") : HUGE_VAL;
+    } else {
+        const uint64_t b = (uint64_t)(ex - 15 + 1023) << 52 | (uint64_t)m << 42;
+        std::memcpy(&v, &b, 8);
+    }
+    return (h & 0x8000) ? -v : v;
+}
+
+// v ~= hi + lo with hi = fp16(v) and lo = fp16(v - hi), each a correctly
+// rounded conversion from double.
 void host_split16(double v, _Float16 &hi, _Float16 &lo) {
-    hi = (_Float16)(float)v;
-    lo = (_Float16)(float)(v - (double)hi);
+    const uint16_t hb = half_bits_rne(v);
+    const uint16_t lb = half_bits_rne(v - half_bits_to_double(hb));
+    std::memcpy(&hi, &hb, 2);
+    std::memcpy(&lo, &lb, 2);
 }
 
 template <typename T>
@@ -321,9 +369,36 @@ void build_tables(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, i
     }
     const double *h0 = sq.coef;
     const double g = sq.gamma;
+    // B-side scale of each higher-order term: a power of two, so the product
+    // equals ldexp(., kHOOut - aexp) exactly
+    double bscale[12];
+    for (int q = 0; q < NHO; ++q) bscale[q] = std::ldexp(1.0, kHOOut - ps.aexp[q]);
     auto &kr = ts.kr, &k64 = ts.k64, &kr0 = ts.kr0, &kend = ts.kend, &anc = ts.anc;
     auto &bho = ts.bho;
     auto &tk = ts.tk;
+    {
+        // Size every table once: growing them segment by segment copies
+        // everything accumulated so far at each reallocation.
+        int64_t tiles = 0, nab = 0;
+        for (int64_t b = 0; b < np;) {
+            if (rf[b]) {
+                ++b;
+                continue;
+            }
+            int64_t e = b, nadc = 0;
+            while (e + 1 < np && !rf[e + 1]) ++e;
+            for (int64_t n = pf[b] + 1; n <= pl[e]; ++n) nadc += sq.adc[(size_t)n] ? 1 : 0;
+            tiles += (nadc + 15) / 16;
+            nab += ((nadc + 15) / 16 + kAnchorTiles - 1) / kAnchorTiles;
+            b = e + 1;
+        }
+        kr.reserve((size_t)(tiles * NI * 64));
+        k64.reserve((size_t)(tiles * 64));
+        bho.reserve((size_t)(tiles * NMH * 64));
+        tk.reserve((size_t)(tiles * 16));
+        kr0.reserve((size_t)(tiles * 16));
+        anc.reserve((size_t)(nab * NHO));
+    }
     int64_t row = 0, b = 0;
     while (b < np) {
         Segment s;
@@ -379,16 +454,18 @@ void build_tables(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, i
                 k64[(size_t)(s.k64_off + tau * 64 + 16 * q + j)] = pos64[q] >= 0 ? Kr[pos64[q]] : 0.0;
             if (k % (16 * kAnchorTiles) == 0)
                 for (int q = 0; q < NHO; ++q) anc[(size_t)(s.anc_off + ab * NHO + q)] = Kr[posho[q]];
-            for (int slot = 0; slot < 3 * NHO; ++slot) {
-                const int term = slot % NHO, kind = slot / NHO;
-                const double bval = ldexp(Kr[posho[term]] - anc[(size_t)(s.anc_off + ab * NHO + term)],
-                                          kHOOut - ps.aexp[term]);
+            // slots term, NHO + term, 2 NHO + term hold the B-side hi, hi, lo
+            // of the term's split offset (the A side holds hi, lo, hi)
+            for (int term = 0; term < NHO; ++term) {
+                const double bval = (Kr[posho[term]] - anc[(size_t)(s.anc_off + ab * NHO + term)]) * bscale[term];
                 bmax = std::max(bmax, std::fabs(bval));
                 _Float16 hi, lo;
                 host_split16(bval, hi, lo);
-                const int m = slot / 32, sl = slot % 32;
-                bho[(size_t)(s.bho_off + (tau * NMH + m) * 64 + 16 * (sl / 8) + j)][sl % 8] =
-                    kind == 2 ? lo : hi;
+                for (int kind = 0; kind < 3; ++kind) {
+                    const int slot = kind * NHO + term, m = slot / 32, sl = slot % 32;
+                    bho[(size_t)(s.bho_off + (tau * NMH + m) * 64 + 16 * (sl / 8) + j)][sl % 8] =
+                        kind == 2 ? lo : hi;
+                }
             }
             tk[(size_t)(s.s_off + k)] = (float)t;
             kr0[(size_t)(s.s_off + k)] = -g * I0;

@@ -375,6 +375,81 @@ hipError_t launch_fill_complex(hom_plan *p, void *out, int64_t rows, int64_t col
     return hipGetLastError();
 }
 
+// ---- input statistics of the uploads ----
+// Bit patterns of nonnegative doubles order like their values, so an unsigned
+// 64-bit atomicMax keeps the largest value.
+__device__ inline void atomic_max_nonneg(unsigned long long *a, double v) {
+    atomicMax(a, (unsigned long long)__double_as_longlong(v));
+}
+
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    return v;
+}
+
+// out[j] (j = 1..16): max over spins of |basis term j|, evaluated as on the
+// host (no contraction), so the mixed path picks the same operand exponents.
+template <typename T>
+__global__ void k_basis_absmax(int64_t ns, const T *__restrict__ x, const T *__restrict__ y,
+                               const T *__restrict__ z, const T *__restrict__ dw,
+                               unsigned long long *__restrict__ out) {
+#pragma clang fp contract(off)
+    double m[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) m[j] = 0.0;
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < ns;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        const double xs = x[s], ys = y[s], zs = z[s], ws = dw[s];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) m[j] = fmax(m[j], fabs(basis_term(j + 1, xs, ys, zs, ws)));
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const double v = wave_max(m[j]);
+        if ((threadIdx.x & 63) == 0) atomic_max_nonneg(out + j + 1, v);
+    }
+}
+
+// out[c]: max over spins of |csm[s, c]|^2 (column-major, column stride ns).
+template <typename T>
+__global__ void k_csm_colmax2(int64_t ns, const Cx<T> *__restrict__ csm, unsigned long long *__restrict__ out) {
+    const int64_t c = blockIdx.y;
+    const Cx<T> *col = csm + c * ns;
+    double m = 0.0;
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < ns;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        const double re = col[s].re, im = col[s].im;
+        m = fmax(m, re * re + im * im);
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(out + c, m);
+}
+
+hipError_t launch_basis_absmax(hom_plan *p, unsigned long long *out) {
+    const unsigned grid = (unsigned)std::min<int64_t>(grid_for(p->ns, 256), 1024);
+    if (p->is_f32())
+        hipLaunchKernelGGL((k_basis_absmax<float>), dim3(grid), dim3(256), 0, p->stream, p->ns,
+                           (const float *)p->d.x, (const float *)p->d.y, (const float *)p->d.z,
+                           (const float *)p->d.dw, out);
+    else
+        hipLaunchKernelGGL((k_basis_absmax<double>), dim3(grid), dim3(256), 0, p->stream, p->ns,
+                           (const double *)p->d.x, (const double *)p->d.y, (const double *)p->d.z,
+                           (const double *)p->d.dw, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_csm_colmax2(hom_plan *p, unsigned long long *out) {
+    const dim3 grid((unsigned)std::min<int64_t>(grid_for(p->ns, 256), 64), (unsigned)p->nc);
+    if (p->is_f32())
+        hipLaunchKernelGGL((k_csm_colmax2<float>), grid, dim3(256), 0, p->stream, p->ns,
+                           (const Cx<float> *)p->d.csm, out);
+    else
+        hipLaunchKernelGGL((k_csm_colmax2<double>), grid, dim3(256), 0, p->stream, p->ns,
+                           (const Cx<double> *)p->d.csm, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_reset_state(hom_plan *p) {
     if (p->ns == 0) return hipSuccess;
     if (p->is_f32())
