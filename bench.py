@@ -269,12 +269,15 @@ def main():
                "sample": f"every {args.cpu_subsample}th spin ({len(sub)} spins) x all {total_adc} "
                          f"samples, {nc} coils, f64, {threads} threads, {cdt:.1f}s"}
         log(f"cpu oracle: {cpu['value']:.4e} spin-samples/s")
-        sub1 = obj[::args.cpu_subsample * 8]
+        # one thread on about 1.5e8 spin-samples (a few seconds)
+        stride1 = max(1, -(-ns * total_adc // 150_000_000))
+        sub1 = obj[::stride1]
+        log(f"oracle on {len(sub1)} spins x {total_adc} samples, 1 thread")
         a = time.perf_counter()
         oracle.run(sub1, seq, parts, method.flags, GAMMA, n_threads=1)
         c1 = time.perf_counter() - a
         cpu["single_thread"] = {"value": len(sub1) * total_adc / c1, "unit": "spin-samples/s",
-                                "sample": f"every {args.cpu_subsample * 8}th spin ({len(sub1)} "
+                                "sample": f"every {stride1}th spin ({len(sub1)} "
                                           f"spins) x all {total_adc} samples, 1 thread, {c1:.1f}s"}
         log(f"cpu oracle, 1 thread: {cpu['single_thread']['value']:.4e} spin-samples/s")
         with Plan("mixed", len(sub), nc, seq.n_nodes, max_adc) as ps:
