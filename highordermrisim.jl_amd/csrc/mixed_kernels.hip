@@ -392,7 +392,6 @@ void build_tables(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, i
             nab += ((nadc + 15) / 16 + kAnchorTiles - 1) / kAnchorTiles;
             b = e + 1;
         }
-        kr.reserve((size_t)(tiles * NI * 64));
         k64.reserve((size_t)(tiles * 64));
         bho.reserve((size_t)(tiles * NMH * 64));
         tk.reserve((size_t)(tiles * 16));
@@ -424,13 +423,11 @@ void build_tables(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, i
         s.tiles = (s.nadc + 15) / 16;
         s.nsb = (s.tiles + kBlockTiles - 1) / kBlockTiles;
         s.nab = (s.tiles + kAnchorTiles - 1) / kAnchorTiles;
-        s.kr_off = (int64_t)kr.size();
         s.k64_off = (int64_t)k64.size();
         s.bho_off = (int64_t)bho.size();
         s.s_off = (int64_t)tk.size();
         s.kend_off = (int64_t)kend.size();
         s.anc_off = (int64_t)anc.size();
-        kr.resize(kr.size() + (size_t)(s.tiles * NI * 64), 0.0);
         k64.resize(k64.size() + (size_t)(s.tiles * 64), 0.0);
         bho.resize(bho.size() + (size_t)(s.tiles * NMH * 64), half8_t{});
         tk.resize(tk.size() + (size_t)(s.tiles * 16), 0.0f);
@@ -448,8 +445,6 @@ void build_tables(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, i
             if (!sq.adc[(size_t)n]) continue;
             for (int q = 0; q < tl.n; ++q) Kr[q] = tl.code[q] == kTermDW ? -t / kTwoPi : -g * I[q];
             const int64_t tau = k / 16, j = k % 16, ab = k / (16 * kAnchorTiles);
-            for (int q = 0; q < tl.n; ++q)
-                kr[(size_t)(s.kr_off + (tau * NI + q / 4) * 64 + 16 * (q % 4) + j)] = Kr[q];
             for (int q = 0; q < 4; ++q)
                 k64[(size_t)(s.k64_off + tau * 64 + 16 * q + j)] = pos64[q] >= 0 ? Kr[pos64[q]] : 0.0;
             if (k % (16 * kAnchorTiles) == 0)
@@ -472,6 +467,27 @@ void build_tables(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, i
             ++k;
         }
         s.fallback = !(bmax <= kHOBMax);
+        if (s.fallback) {
+            // All-FP64 table of every term, only for segments that need it:
+            // the same integration again, writing Kr of each sample.
+            s.kr_off = (int64_t)kr.size();
+            kr.resize(kr.size() + (size_t)(s.tiles * NI * 64), 0.0);
+            std::vector<double> I2(tl.n, 0.0);
+            double t2 = 0.0;
+            int64_t k2 = 0;
+            for (int64_t n = s.first + 1; n <= s.last; ++n) {
+                const double dt = sq.dt[(size_t)(n - 1)];
+                for (int q = 0; q < tl.n; ++q)
+                    if (wave[q]) I2[q] += (wave[q][n - 1] + wave[q][n]) * (dt / 2.0);
+                t2 += dt;
+                if (!sq.adc[(size_t)n]) continue;
+                const int64_t tau = k2 / 16, j = k2 % 16;
+                for (int q = 0; q < tl.n; ++q)
+                    kr[(size_t)(s.kr_off + (tau * NI + q / 4) * 64 + 16 * (q % 4) + j)] =
+                        tl.code[q] == kTermDW ? -t2 / kTwoPi : -g * I2[q];
+                ++k2;
+            }
+        }
         for (int q = 0; q < tl.n; ++q) kend.push_back(tl.code[q] == kTermDW ? -t / kTwoPi : -g * I[q]);
         kend.push_back(-g * I0);
         s.T = t;
@@ -682,7 +698,7 @@ int mixed_run(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *
                 o.partials = W.partials;
                 o.basisA = W.basisA;
                 o.th0 = W.th0;
-                o.kr = W.kr + s.kr_off;
+                o.kr = s.fallback ? W.kr + s.kr_off : nullptr;
                 ScopedTimer timer(p, HOM_K_MIXED_SEGMENT);
                 rc = check_hip(p, launch_main(ncb, s.fallback, W.split.nmh, W.NI, p->stream, o, s.nsb,
                                               nwork),
