@@ -23,6 +23,7 @@
 #include "mixed_common.h"
 
 #include <cstring>
+#include <thread>
 
 #ifndef HOM_GPT
 #define HOM_GPT 256
@@ -344,6 +345,26 @@ void free_dev(T *&q) {
 //   bho (FP16 offsets):      tile tau, MFMA m, lane l, element i -> K slot
 //                            32m + 8(l/16) + i at sample l%16
 //   anc: per anchor block, the higher-order Kr at the block's first sample.
+// Runs fill(k0, k1, max) over [0, n) in up to 8 host threads (chunks of at
+// least 2048 items) and returns the largest of the threads' maxima.
+template <typename F>
+double parallel_max(int64_t n, F &&fill) {
+    const int64_t hw = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
+    const int64_t nth = std::max<int64_t>(1, std::min<int64_t>({8, hw, n / 2048}));
+    std::vector<double> m((size_t)nth, 0.0);
+    if (nth == 1) {
+        fill(0, n, m[0]);
+        return m[0];
+    }
+    std::vector<std::thread> pool;
+    pool.reserve((size_t)nth - 1);
+    for (int64_t i = 1; i < nth; ++i)
+        pool.emplace_back([&, i] { fill(n * i / nth, n * (i + 1) / nth, m[(size_t)i]); });
+    fill(0, n / nth, m[0]);
+    for (auto &th : pool) th.join();
+    return *std::max_element(m.begin(), m.end());
+}
+
 void build_tables(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, int NI, int flags,
                   const int64_t *pf, const int64_t *pl, const uint8_t *rf, int64_t np,
                   std::vector<Segment> &segs, TableSet &ts) {
@@ -433,59 +454,71 @@ void build_tables(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, i
         tk.resize(tk.size() + (size_t)(s.tiles * 16), 0.0f);
         kr0.resize(kr0.size() + (size_t)(s.tiles * 16), 0.0);
         anc.resize(anc.size() + (size_t)(s.nab * NHO), 0.0);
-        std::vector<double> I(tl.n, 0.0), Kr(tl.n, 0.0);
-        double I0 = 0.0, t = 0.0, bmax = 0.0;
+        // Phase 1 (sequential): the trapezoid integrals at every acquired
+        // node (Kr of each term, the time, the zeroth order).
+        const int nt = tl.n;
+        std::vector<double> KrS((size_t)(s.nadc * nt)), tS((size_t)s.nadc), i0S((size_t)s.nadc);
+        std::vector<double> I(nt, 0.0);
+        double I0 = 0.0, t = 0.0;
         int64_t k = 0;
         for (int64_t n = s.first + 1; n <= s.last; ++n) {
             const double dt = sq.dt[(size_t)(n - 1)];
-            for (int q = 0; q < tl.n; ++q)
+            for (int q = 0; q < nt; ++q)
                 if (wave[q]) I[q] += (wave[q][n - 1] + wave[q][n]) * (dt / 2.0);
             if (ho0) I0 += (h0[n - 1] + h0[n]) * (dt / 2.0);
             t += dt;
             if (!sq.adc[(size_t)n]) continue;
-            for (int q = 0; q < tl.n; ++q) Kr[q] = tl.code[q] == kTermDW ? -t / kTwoPi : -g * I[q];
-            const int64_t tau = k / 16, j = k % 16, ab = k / (16 * kAnchorTiles);
-            for (int q = 0; q < 4; ++q)
-                k64[(size_t)(s.k64_off + tau * 64 + 16 * q + j)] = pos64[q] >= 0 ? Kr[pos64[q]] : 0.0;
-            if (k % (16 * kAnchorTiles) == 0)
-                for (int q = 0; q < NHO; ++q) anc[(size_t)(s.anc_off + ab * NHO + q)] = Kr[posho[q]];
-            // slots term, NHO + term, 2 NHO + term hold the B-side hi, hi, lo
-            // of the term's split offset (the A side holds hi, lo, hi)
-            for (int term = 0; term < NHO; ++term) {
-                const double bval = (Kr[posho[term]] - anc[(size_t)(s.anc_off + ab * NHO + term)]) * bscale[term];
-                bmax = std::max(bmax, std::fabs(bval));
-                _Float16 hi, lo;
-                host_split16(bval, hi, lo);
-                for (int kind = 0; kind < 3; ++kind) {
-                    const int slot = kind * NHO + term, m = slot / 32, sl = slot % 32;
-                    bho[(size_t)(s.bho_off + (tau * NMH + m) * 64 + 16 * (sl / 8) + j)][sl % 8] =
-                        kind == 2 ? lo : hi;
-                }
-            }
-            tk[(size_t)(s.s_off + k)] = (float)t;
-            kr0[(size_t)(s.s_off + k)] = -g * I0;
+            double *kr_k = &KrS[(size_t)(k * nt)];
+            for (int q = 0; q < nt; ++q) kr_k[q] = tl.code[q] == kTermDW ? -t / kTwoPi : -g * I[q];
+            tS[(size_t)k] = t;
+            i0S[(size_t)k] = I0;
             ++k;
         }
+        // higher-order anchors: the values at the first sample of each 256-sample block
+        for (int64_t ab = 0; ab < s.nab; ++ab)
+            for (int q = 0; q < NHO; ++q)
+                anc[(size_t)(s.anc_off + ab * NHO + q)] = KrS[(size_t)(ab * 16 * kAnchorTiles * nt + posho[q])];
+        // Phase 2 (samples independent, split over threads): FP64 and
+        // split-FP16 table entries, and the largest scaled offset.
+        _Float16 *const bho16 = reinterpret_cast<_Float16 *>(bho.data());
+        auto fill = [&](int64_t k0, int64_t k1, double &bm_out) {
+            double bm = 0.0;  // local: the threads' results share a cache line
+            for (int64_t kk = k0; kk < k1; ++kk) {
+                const double *kr_k = &KrS[(size_t)(kk * nt)];
+                const int64_t tau = kk / 16, j = kk % 16, ab = kk / (16 * kAnchorTiles);
+                for (int q = 0; q < 4; ++q)
+                    k64[(size_t)(s.k64_off + tau * 64 + 16 * q + j)] = pos64[q] >= 0 ? kr_k[pos64[q]] : 0.0;
+                // slots term, NHO + term, 2 NHO + term hold the B-side hi, hi,
+                // lo of the term's split offset (the A side holds hi, lo, hi)
+                for (int term = 0; term < NHO; ++term) {
+                    const double bval =
+                        (kr_k[posho[term]] - anc[(size_t)(s.anc_off + ab * NHO + term)]) * bscale[term];
+                    bm = std::max(bm, std::fabs(bval));
+                    _Float16 hi, lo;
+                    host_split16(bval, hi, lo);
+                    for (int kind = 0; kind < 3; ++kind) {
+                        const int slot = kind * NHO + term, m = slot / 32, sl = slot % 32;
+                        // element sl % 8 of the 8-half chunk, stored as a scalar
+                        bho16[(size_t)(s.bho_off + (tau * NMH + m) * 64 + 16 * (sl / 8) + j) * 8 + sl % 8] =
+                            kind == 2 ? lo : hi;
+                    }
+                }
+                tk[(size_t)(s.s_off + kk)] = (float)tS[(size_t)kk];
+                kr0[(size_t)(s.s_off + kk)] = -g * i0S[(size_t)kk];
+            }
+            bm_out = bm;
+        };
+        const double bmax = parallel_max(s.nadc, fill);
         s.fallback = !(bmax <= kHOBMax);
         if (s.fallback) {
-            // All-FP64 table of every term, only for segments that need it:
-            // the same integration again, writing Kr of each sample.
+            // All-FP64 table of every term, only for segments that need it.
             s.kr_off = (int64_t)kr.size();
             kr.resize(kr.size() + (size_t)(s.tiles * NI * 64), 0.0);
-            std::vector<double> I2(tl.n, 0.0);
-            double t2 = 0.0;
-            int64_t k2 = 0;
-            for (int64_t n = s.first + 1; n <= s.last; ++n) {
-                const double dt = sq.dt[(size_t)(n - 1)];
-                for (int q = 0; q < tl.n; ++q)
-                    if (wave[q]) I2[q] += (wave[q][n - 1] + wave[q][n]) * (dt / 2.0);
-                t2 += dt;
-                if (!sq.adc[(size_t)n]) continue;
-                const int64_t tau = k2 / 16, j = k2 % 16;
-                for (int q = 0; q < tl.n; ++q)
+            for (int64_t kk = 0; kk < s.nadc; ++kk) {
+                const int64_t tau = kk / 16, j = kk % 16;
+                for (int q = 0; q < nt; ++q)
                     kr[(size_t)(s.kr_off + (tau * NI + q / 4) * 64 + 16 * (q % 4) + j)] =
-                        tl.code[q] == kTermDW ? -t2 / kTwoPi : -g * I2[q];
-                ++k2;
+                        KrS[(size_t)(kk * nt + q)];
             }
         }
         for (int q = 0; q < tl.n; ++q) kend.push_back(tl.code[q] == kTermDW ? -t / kTwoPi : -g * I[q]);

@@ -298,12 +298,72 @@ struct Arena {
     }
 };
 
-constexpr int64_t kMaxBlock = 4096;  // spins per block of the range-max tables
+// Range maxima over spins of w values per spin (basis terms, coil
+// magnitudes), from tables at two block sizes: sub-blocks of kSub spins and
+// blocks of kMaxBlock spins.  A query reads at most 2 (kSub - 1) spins from
+// the per-spin source, at most 2 (kMaxBlock / kSub - 1) sub-blocks and the
+// whole blocks between them.
+constexpr int64_t kSub = 64;
+constexpr int64_t kMaxBlock = 4096;
+constexpr int64_t kSubPerBlock = kMaxBlock / kSub;
+
+struct RangeMax {
+    int64_t w = 0;
+    std::vector<double> sub, blk;  // (sub-blocks | blocks) x w
+
+    // value(i, k) >= 0: value k of spin i; built column by column (k outer).
+    template <typename F>
+    void build(int64_t ns, int64_t w_, F value) {
+        w = w_;
+        const int64_t nsub = (ns + kSub - 1) / kSub, nblk = (ns + kMaxBlock - 1) / kMaxBlock;
+        sub.assign((size_t)(nsub * w), 0.0);
+        blk.assign((size_t)(nblk * w), 0.0);
+        for (int64_t k = 0; k < w; ++k)
+            for (int64_t i = 0; i < ns; ++i) {
+                double &m = sub[(size_t)((i / kSub) * w + k)];
+                m = std::max(m, value(i, k));
+            }
+        for (int64_t b = 0; b < nsub; ++b)
+            for (int64_t k = 0; k < w; ++k) {
+                double &m = blk[(size_t)((b / kSubPerBlock) * w + k)];
+                m = std::max(m, sub[(size_t)(b * w + k)]);
+            }
+    }
+
+    // out[k] = max over spins [a, b) of value k, for k < wq <= w.
+    template <typename F>
+    void query(int64_t a, int64_t b, int64_t wq, F value, double *out) const {
+        for (int64_t k = 0; k < wq; ++k) out[k] = 0.0;
+        auto direct = [&](int64_t lo, int64_t hi) {
+            for (int64_t i = lo; i < hi; ++i)
+                for (int64_t k = 0; k < wq; ++k) out[k] = std::max(out[k], value(i, k));
+        };
+        auto table = [&](const std::vector<double> &t, int64_t lo, int64_t hi) {
+            for (int64_t q = lo; q < hi; ++q)
+                for (int64_t k = 0; k < wq; ++k) out[k] = std::max(out[k], t[(size_t)(q * w + k)]);
+        };
+        const int64_t s0 = (a + kSub - 1) / kSub, s1 = b / kSub;  // whole sub-blocks [s0, s1)
+        if (s0 >= s1) {
+            direct(a, b);
+            return;
+        }
+        direct(a, s0 * kSub);
+        direct(s1 * kSub, b);
+        const int64_t b0 = (s0 + kSubPerBlock - 1) / kSubPerBlock, b1 = s1 / kSubPerBlock;
+        if (b0 >= b1) {
+            table(sub, s0, s1);
+            return;
+        }
+        table(sub, s0, b0 * kSubPerBlock);
+        table(sub, b1 * kSubPerBlock, s1);
+        table(blk, b0, b1);
+    }
+};
 
 struct ResPhantom {
     int64_t ns = 0;
     double *d = nullptr;  // x, y, z, rho, T1, T2, dw (ns each)
-    std::vector<double> bmax;  // per block of kMaxBlock spins: max |basis term j|, j = 0..16
+    RangeMax bmax;             // max |basis term j|, j = 0..16
     std::vector<double> hx, hy, hz, hdw;
     int refs = 0;
     bool released = false;
@@ -313,7 +373,7 @@ struct ResPhantom {
 struct ResCsm {
     int64_t phantom = 0, nc = 0, ns = 0;
     double2 *d = nullptr;          // ns x nc, column stride ns
-    std::vector<double> cmax;      // per block: max |csm| per coil
+    RangeMax cmax2;                // max |csm|^2 per coil
     std::vector<double> host;      // interleaved copy for the partial ends of range queries
     int refs = 0;
     bool released = false;
@@ -424,44 +484,25 @@ int hip_fail(hom_server *s, hipError_t e, const char *what) {
 }
 
 // Max of |basis term j| (j = 0..16) over spins [a, b) of a phantom.
+double basis_abs(const ResPhantom &ph, int64_t i, int64_t j) {
+    return std::fabs(basis_term((int)j, ph.hx[(size_t)i], ph.hy[(size_t)i], ph.hz[(size_t)i], ph.hdw[(size_t)i]));
+}
+
 void range_basis_max(const ResPhantom &ph, int64_t a, int64_t b, double out[17]) {
-    for (int j = 0; j <= 16; ++j) out[j] = 0.0;
-    auto direct = [&](int64_t lo, int64_t hi) {
-        for (int64_t s = lo; s < hi; ++s)
-            for (int j = 1; j <= 16; ++j)
-                out[j] = std::max(out[j], std::fabs(basis_term(j, ph.hx[s], ph.hy[s], ph.hz[s], ph.hdw[s])));
-    };
-    const int64_t b0 = (a + kMaxBlock - 1) / kMaxBlock, b1 = b / kMaxBlock;
-    if (b0 >= b1) {
-        direct(a, b);
-        return;
-    }
-    direct(a, b0 * kMaxBlock);
-    for (int64_t k = b0; k < b1; ++k)
-        for (int j = 0; j <= 16; ++j) out[j] = std::max(out[j], ph.bmax[(size_t)(k * 17 + j)]);
-    direct(b1 * kMaxBlock, b);
+    ph.bmax.query(a, b, 17, [&](int64_t i, int64_t j) { return basis_abs(ph, i, j); }, out);
+}
+
+// |csm[i, c]|^2 of a resident CSM.
+double csm_abs2(const ResCsm &cs, int64_t i, int64_t c) {
+    const double *v = &cs.host[(size_t)(2 * (i + c * cs.ns))];
+    return v[0] * v[0] + v[1] * v[1];
 }
 
 // Max |csm| per coil c < nc over spins [a, b).
 void range_csm_max(const ResCsm &cs, int64_t a, int64_t b, int64_t nc, std::vector<double> &out) {
     out.assign((size_t)nc, 0.0);
-    auto direct = [&](int64_t lo, int64_t hi) {
-        for (int64_t c = 0; c < nc; ++c)
-            for (int64_t s = lo; s < hi; ++s) {
-                const double *v = &cs.host[(size_t)(2 * (s + c * cs.ns))];
-                out[(size_t)c] = std::max(out[(size_t)c], std::hypot(v[0], v[1]));
-            }
-    };
-    const int64_t b0 = (a + kMaxBlock - 1) / kMaxBlock, b1 = b / kMaxBlock;
-    if (b0 >= b1) {
-        direct(a, b);
-        return;
-    }
-    direct(a, b0 * kMaxBlock);
-    for (int64_t k = b0; k < b1; ++k)
-        for (int64_t c = 0; c < nc; ++c)
-            out[(size_t)c] = std::max(out[(size_t)c], cs.cmax[(size_t)(k * cs.nc + c)]);
-    direct(b1 * kMaxBlock, b);
+    cs.cmax2.query(a, b, nc, [&](int64_t i, int64_t c) { return csm_abs2(cs, i, c); }, out.data());
+    for (double &m : out) m = std::sqrt(m);
 }
 
 struct Layout {
@@ -1197,13 +1238,8 @@ int hom_server_put_phantom(hom_server *s, int64_t ns, const double *x, const dou
     ph->hy.assign(y, y + ns);
     ph->hz.assign(z, z + ns);
     ph->hdw.assign(dw, dw + ns);
-    const int64_t nb = (ns + kMaxBlock - 1) / kMaxBlock;
-    ph->bmax.assign((size_t)(nb * 17), 0.0);
-    for (int64_t i = 0; i < ns; ++i)
-        for (int j = 1; j <= 16; ++j) {
-            double &m = ph->bmax[(size_t)((i / kMaxBlock) * 17 + j)];
-            m = std::max(m, std::fabs(basis_term(j, x[i], y[i], z[i], dw[i])));
-        }
+    const ResPhantom &phr = *ph;
+    ph->bmax.build(ns, 17, [&](int64_t i, int64_t j) { return basis_abs(phr, i, j); });
     std::lock_guard<std::mutex> lk(s->mu);
     *id = s->next_id++;
     s->phantoms[*id] = std::move(ph);
@@ -1239,14 +1275,8 @@ int hom_server_put_csm(hom_server *s, int64_t phantom, int64_t nc, const double 
         hipFree(cs->d);
         return hip_fail(s, e, "CSM upload");
     }
-    const int64_t nb = (ns + kMaxBlock - 1) / kMaxBlock;
-    cs->cmax.assign((size_t)(nb * nc), 0.0);
-    for (int64_t c = 0; c < nc; ++c)
-        for (int64_t i = 0; i < ns; ++i) {
-            const double *v = &cs->host[(size_t)(2 * (i + c * ns))];
-            double &m = cs->cmax[(size_t)((i / kMaxBlock) * nc + c)];
-            m = std::max(m, std::hypot(v[0], v[1]));
-        }
+    const ResCsm &csr = *cs;
+    cs->cmax2.build(ns, nc, [&](int64_t i, int64_t c) { return csm_abs2(csr, i, c); });
     std::lock_guard<std::mutex> lk(s->mu);
     *id = s->next_id++;
     s->csms[*id] = std::move(cs);
