@@ -185,8 +185,8 @@ struct hom_discrete {
 
 extern "C" {
 
-int hom_discretize(const hom_block *blocks, int64_t n_blocks, double dt_max, double dt_rf,
-                   int nominal_in_h, hom_discrete **out, int64_t *n_nodes) {
+static int hom_discretize_impl(const hom_block *blocks, int64_t n_blocks, double dt_max, double dt_rf,
+                               int nominal_in_h, hom_discrete **out, int64_t *n_nodes) {
     if (!out || !n_nodes || (n_blocks > 0 && !blocks) || n_blocks < 0) return HOM_ERR_ARGUMENT;
     *out = nullptr;
     if (!(dt_max > 0.0) || !(dt_rf > 0.0)) return HOM_ERR_ARGUMENT;
@@ -217,9 +217,9 @@ int hom_discretize(const hom_block *blocks, int64_t n_blocks, double dt_max, dou
     return HOM_OK;
 }
 
-int hom_discrete_copy(const hom_discrete *d, double *t, double *gx, double *gy, double *gz,
-                      double *b1, double *df, double *psi, uint8_t *adc, uint8_t *excitation,
-                      double *h) {
+static int hom_discrete_copy_impl(const hom_discrete *d, double *t, double *gx, double *gy, double *gz,
+                                  double *b1, double *df, double *psi, uint8_t *adc, uint8_t *excitation,
+                                  double *h) {
     if (!d) return HOM_ERR_ARGUMENT;
     const int64_t n = (int64_t)d->nodes.size();
     for (int64_t i = 0; i < n; ++i) {
@@ -246,6 +246,23 @@ int hom_discrete_copy(const hom_discrete *d, double *t, double *gx, double *gy, 
 int hom_discrete_free(hom_discrete *d) {
     delete d;
     return HOM_OK;
+}
+
+// Entry points that allocate host memory or start threads: no exception
+// crosses the C ABI.
+int hom_discretize(const hom_block *blocks, int64_t n_blocks, double dt_max, double dt_rf,
+                   int nominal_in_h, hom_discrete **out, int64_t *n_nodes) {
+    return hom::guarded_host([&] {
+        return hom_discretize_impl(blocks, n_blocks, dt_max, dt_rf, nominal_in_h, out, n_nodes);
+    });
+}
+
+int hom_discrete_copy(const hom_discrete *d, double *t, double *gx, double *gy, double *gz,
+                      double *b1, double *df, double *psi, uint8_t *adc, uint8_t *excitation,
+                      double *h) {
+    return hom::guarded_host([&] {
+        return hom_discrete_copy_impl(d, t, gx, gy, gz, b1, df, psi, adc, excitation, h);
+    });
 }
 
 }  // extern "C"

@@ -157,6 +157,24 @@ ScopedTimer::~ScopedTimer() {
 
 }  // namespace hom
 
+namespace {
+// Runs f() for a plan entry point (p may be NULL while a plan is created): a
+// failed host allocation becomes HOM_ERR_MEMORY, any other exception
+// HOM_ERR_STATE with its text in hom_last_error.
+template <typename F>
+int guarded(hom_plan *p, F &&f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc &) {
+        return set_error(p, HOM_ERR_MEMORY, "Host memory allocation failed.");
+    } catch (const std::exception &e) {
+        return set_error(p, HOM_ERR_STATE, std::string("Internal error: ") + e.what());
+    } catch (...) {
+        return set_error(p, HOM_ERR_STATE, "Internal error.");
+    }
+}
+}  // namespace
+
 extern "C" {
 
 int hom_abi_version(void) { return HOM_ABI_VERSION; }
@@ -173,9 +191,9 @@ const char *hom_last_error(const hom_plan *plan) {
     return plan ? plan->err.c_str() : g_create_error.c_str();
 }
 
-int hom_plan_create(hom_plan **out, int device, int precision, int64_t n_spins, int64_t n_coils,
-                    int64_t n_nodes, int64_t max_adc, int64_t max_rf_adc, double gamma,
-                    int coil_combination) {
+static int hom_plan_create_impl(hom_plan **out, int device, int precision, int64_t n_spins, int64_t n_coils,
+                                int64_t n_nodes, int64_t max_adc, int64_t max_rf_adc, double gamma,
+                                int coil_combination) {
     g_create_error.clear();
     if (!out) return set_error(nullptr, HOM_ERR_ARGUMENT, "The plan output pointer is NULL.");
     *out = nullptr;
@@ -251,8 +269,8 @@ int hom_plan_destroy(hom_plan *p) {
     return HOM_OK;
 }
 
-int hom_upload_phantom(hom_plan *p, const double *x, const double *y, const double *z,
-                       const double *rho, const double *T1, const double *T2, const double *dw) {
+static int hom_upload_phantom_impl(hom_plan *p, const double *x, const double *y, const double *z,
+                                   const double *rho, const double *T1, const double *T2, const double *dw) {
     if (!p) return HOM_ERR_ARGUMENT;
     if (p->ns > 0 && (!x || !y || !z || !rho || !T1 || !T2 || !dw))
         return set_error(p, HOM_ERR_ARGUMENT, "Every phantom field needs a host pointer.");
@@ -285,7 +303,7 @@ int hom_upload_phantom(hom_plan *p, const double *x, const double *y, const doub
     return hom_reset_state(p);
 }
 
-int hom_upload_csm(hom_plan *p, const double *csm, int64_t ld) {
+static int hom_upload_csm_impl(hom_plan *p, const double *csm, int64_t ld) {
     if (!p) return HOM_ERR_ARGUMENT;
     if (ld < std::max<int64_t>(p->ns, 1))
         return set_error(p, HOM_ERR_DIMENSION, "The CSM leading dimension is smaller than the spin count.");
@@ -334,7 +352,7 @@ int hom_upload_csm(hom_plan *p, const double *csm, int64_t ld) {
     return HOM_OK;
 }
 
-int hom_upload_sequence(hom_plan *p, const hom_sequence_host *seq) {
+static int hom_upload_sequence_impl(hom_plan *p, const hom_sequence_host *seq) {
     if (!p) return HOM_ERR_ARGUMENT;
     if (!seq) return set_error(p, HOM_ERR_ARGUMENT, "The sequence pointer is NULL.");
     if (seq->n_nodes != p->nn)
@@ -388,7 +406,7 @@ int hom_reset_state(hom_plan *p) {
     return check_hip(p, launch_reset_state(p), "state reset");
 }
 
-int hom_set_state(hom_plan *p, const double *mxy, const double *mz) {
+static int hom_set_state_impl(hom_plan *p, const double *mxy, const double *mz) {
     if (!p) return HOM_ERR_ARGUMENT;
     if (p->ns > 0 && (!mxy || !mz)) return set_error(p, HOM_ERR_ARGUMENT, "State pointers are NULL.");
     hipSetDevice(p->device);
@@ -397,7 +415,7 @@ int hom_set_state(hom_plan *p, const double *mxy, const double *mz) {
     return rc ? rc : upload_real(p, &p->d.mz, mz, p->ns);
 }
 
-int hom_get_state(hom_plan *p, double *mxy, double *mz) {
+static int hom_get_state_impl(hom_plan *p, double *mxy, double *mz) {
     if (!p) return HOM_ERR_ARGUMENT;
     if (p->ns > 0 && (!mxy || !mz)) return set_error(p, HOM_ERR_ARGUMENT, "State pointers are NULL.");
     hipSetDevice(p->device);
@@ -417,20 +435,20 @@ int hom_get_state(hom_plan *p, double *mxy, double *mz) {
     return rc ? rc : check_hip(p, hipMemcpy(mz, p->d.mz, 8 * ns, hipMemcpyDeviceToHost), "hipMemcpy");
 }
 
-int hom_precession_block(hom_plan *p, int64_t first, int64_t last, int flags, void *out, int64_t ld) {
+static int hom_precession_block_impl(hom_plan *p, int64_t first, int64_t last, int flags, void *out, int64_t ld) {
     if (!p) return HOM_ERR_ARGUMENT;
     hipSetDevice(p->device);
     return block_common(p, first, last, flags, out, ld, false);
 }
 
-int hom_excitation_block(hom_plan *p, int64_t first, int64_t last, int flags, void *out, int64_t ld) {
+static int hom_excitation_block_impl(hom_plan *p, int64_t first, int64_t last, int flags, void *out, int64_t ld) {
     if (!p) return HOM_ERR_ARGUMENT;
     hipSetDevice(p->device);
     return block_common(p, first, last, flags, out, ld, true);
 }
 
-int hom_reduce_signal(hom_plan *p, const void *states, int64_t ld_states, int64_t n_adc, void *out,
-                      int64_t out_ld) {
+static int hom_reduce_signal_impl(hom_plan *p, const void *states, int64_t ld_states, int64_t n_adc, void *out,
+                                  int64_t out_ld) {
     if (!p) return HOM_ERR_ARGUMENT;
     if (!p->has_csm) return set_error(p, HOM_ERR_STATE, "The CSM has not been uploaded.");
     if (n_adc < 0) return set_error(p, HOM_ERR_ARGUMENT, "The ADC count must be nonnegative.");
@@ -445,8 +463,8 @@ int hom_reduce_signal(hom_plan *p, const void *states, int64_t ld_states, int64_
     return check_hip(p, launch_coil_reduce(p, states, ld_states, n_adc, out, out_ld), "coil reduction");
 }
 
-int hom_run(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *rf, int64_t np,
-            int flags) {
+static int hom_run_impl(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *rf, int64_t np,
+                        int flags) {
     if (!p) return HOM_ERR_ARGUMENT;
     int rc = require_inputs(p);
     if (rc) return rc;
@@ -515,7 +533,7 @@ int hom_download_signal(hom_plan *p, void *host, int64_t ld) {
                      "hipMemcpy2D");
 }
 
-int hom_buffer_alloc(hom_plan *p, int64_t bytes, void **dev) {
+static int hom_buffer_alloc_impl(hom_plan *p, int64_t bytes, void **dev) {
     if (!p || !dev || bytes < 0) return HOM_ERR_ARGUMENT;
     hipSetDevice(p->device);
     *dev = nullptr;
@@ -557,7 +575,7 @@ int hom_buffer_download(hom_plan *p, void *host, const void *dev, int64_t bytes)
     return rc ? rc : check_hip(p, hipMemcpy(host, dev, (size_t)bytes, hipMemcpyDeviceToHost), "hipMemcpy");
 }
 
-int hom_plan_set_timing(hom_plan *p, int enable) {
+static int hom_plan_set_timing_impl(hom_plan *p, int enable) {
     if (!p) return HOM_ERR_ARGUMENT;
     hipSetDevice(p->device);
     hipStreamSynchronize(p->stream);
@@ -577,7 +595,7 @@ int hom_plan_mixed_stats(const hom_plan *p, int64_t *segments, int64_t *fp64_seg
     return HOM_OK;
 }
 
-int hom_plan_kernel_time(hom_plan *p, int id, double *total_ms, int64_t *launches) {
+static int hom_plan_kernel_time_impl(hom_plan *p, int id, double *total_ms, int64_t *launches) {
     if (!p || id < 0 || id >= HOM_K_COUNT || !total_ms || !launches) return HOM_ERR_ARGUMENT;
     hipSetDevice(p->device);
     int rc = check_hip(p, hipStreamSynchronize(p->stream), "hipStreamSynchronize");
@@ -594,6 +612,91 @@ int hom_plan_kernel_time(hom_plan *p, int id, double *total_ms, int64_t *launche
     *total_ms = s.folded_ms;
     *launches = s.folded_launches;
     return HOM_OK;
+}
+
+// Entry points that allocate host memory or start threads: no exception
+// crosses the C ABI.
+int hom_upload_phantom(hom_plan *p, const double *x, const double *y, const double *z,
+                       const double *rho, const double *T1, const double *T2, const double *dw) {
+    return guarded(p, [&] {
+        return hom_upload_phantom_impl(p, x, y, z, rho, T1, T2, dw);
+    });
+}
+
+int hom_upload_csm(hom_plan *p, const double *csm, int64_t ld) {
+    return guarded(p, [&] {
+        return hom_upload_csm_impl(p, csm, ld);
+    });
+}
+
+int hom_upload_sequence(hom_plan *p, const hom_sequence_host *seq) {
+    return guarded(p, [&] {
+        return hom_upload_sequence_impl(p, seq);
+    });
+}
+
+int hom_set_state(hom_plan *p, const double *mxy, const double *mz) {
+    return guarded(p, [&] {
+        return hom_set_state_impl(p, mxy, mz);
+    });
+}
+
+int hom_get_state(hom_plan *p, double *mxy, double *mz) {
+    return guarded(p, [&] {
+        return hom_get_state_impl(p, mxy, mz);
+    });
+}
+
+int hom_precession_block(hom_plan *p, int64_t first, int64_t last, int flags, void *out, int64_t ld) {
+    return guarded(p, [&] {
+        return hom_precession_block_impl(p, first, last, flags, out, ld);
+    });
+}
+
+int hom_excitation_block(hom_plan *p, int64_t first, int64_t last, int flags, void *out, int64_t ld) {
+    return guarded(p, [&] {
+        return hom_excitation_block_impl(p, first, last, flags, out, ld);
+    });
+}
+
+int hom_reduce_signal(hom_plan *p, const void *states, int64_t ld_states, int64_t n_adc, void *out,
+                      int64_t out_ld) {
+    return guarded(p, [&] {
+        return hom_reduce_signal_impl(p, states, ld_states, n_adc, out, out_ld);
+    });
+}
+
+int hom_run(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *rf, int64_t np,
+            int flags) {
+    return guarded(p, [&] {
+        return hom_run_impl(p, pf, pl, rf, np, flags);
+    });
+}
+
+int hom_buffer_alloc(hom_plan *p, int64_t bytes, void **dev) {
+    return guarded(p, [&] {
+        return hom_buffer_alloc_impl(p, bytes, dev);
+    });
+}
+
+int hom_plan_set_timing(hom_plan *p, int enable) {
+    return guarded(p, [&] {
+        return hom_plan_set_timing_impl(p, enable);
+    });
+}
+
+int hom_plan_kernel_time(hom_plan *p, int id, double *total_ms, int64_t *launches) {
+    return guarded(p, [&] {
+        return hom_plan_kernel_time_impl(p, id, total_ms, launches);
+    });
+}
+
+int hom_plan_create(hom_plan **out, int device, int precision, int64_t n_spins, int64_t n_coils,
+                    int64_t n_nodes, int64_t max_adc, int64_t max_rf_adc, double gamma,
+                    int coil_combination) {
+    return guarded(nullptr, [&] {
+        return hom_plan_create_impl(out, device, precision, n_spins, n_coils, n_nodes, max_adc, max_rf_adc, gamma, coil_combination);
+    });
 }
 
 }  // extern "C"

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -115,6 +117,20 @@ struct ScopedTimer {
 };
 
 int set_error(hom_plan *p, int code, const std::string &msg);
+
+// Runs f() for an entry point without a handle: a failed host allocation
+// becomes HOM_ERR_MEMORY and any other exception HOM_ERR_STATE, so no C++
+// exception crosses the C ABI.
+template <typename F>
+int guarded_host(F &&f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc &) {
+        return HOM_ERR_MEMORY;
+    } catch (...) {
+        return HOM_ERR_STATE;
+    }
+}
 int check_hip(hom_plan *p, hipError_t e, const char *what);
 
 // ---- plain kernels (plain_kernels.hip) ----

@@ -59,8 +59,8 @@ std::vector<RfEvent> rf_events(const hom_sequence_host *s, double gamma) {
 
 extern "C" {
 
-int hom_kspace_rf_events(const hom_sequence_host *seq, double gamma, int64_t *n_events,
-                         double *flip_deg, int64_t capacity) {
+static int hom_kspace_rf_events_impl(const hom_sequence_host *seq, double gamma, int64_t *n_events,
+                                     double *flip_deg, int64_t capacity) {
     if (!seq || !n_events) return HOM_ERR_ARGUMENT;
     if (seq->n_nodes < 1) return HOM_ERR_DIMENSION;
     if (seq->n_nodes > 1 && (!seq->t || !seq->dt || !seq->b1 || !seq->excitation))
@@ -72,8 +72,8 @@ int hom_kspace_rf_events(const hom_sequence_host *seq, double gamma, int64_t *n_
     return HOM_OK;
 }
 
-int hom_kspace(const hom_sequence_host *seq, double gamma, const uint8_t *skip_rf, int64_t n_skip,
-               double *k_nominal, double *k_nominal_adc, double *k_dfc, double *k_dfc_adc) {
+static int hom_kspace_impl(const hom_sequence_host *seq, double gamma, const uint8_t *skip_rf, int64_t n_skip,
+                           double *k_nominal, double *k_nominal_adc, double *k_dfc, double *k_dfc_adc) {
     if (!seq) return HOM_ERR_ARGUMENT;
     const int64_t n = seq->n_nodes, ni = n - 1;
     if (ni < 1) return HOM_ERR_ARGUMENT;  // "Cannot integrate an empty sequence."
@@ -120,6 +120,22 @@ int hom_kspace(const hom_sequence_host *seq, double gamma, const uint8_t *skip_r
             }
     }
     return HOM_OK;
+}
+
+// Entry points that allocate host memory or start threads: no exception
+// crosses the C ABI.
+int hom_kspace(const hom_sequence_host *seq, double gamma, const uint8_t *skip_rf, int64_t n_skip,
+               double *k_nominal, double *k_nominal_adc, double *k_dfc, double *k_dfc_adc) {
+    return hom::guarded_host([&] {
+        return hom_kspace_impl(seq, gamma, skip_rf, n_skip, k_nominal, k_nominal_adc, k_dfc, k_dfc_adc);
+    });
+}
+
+int hom_kspace_rf_events(const hom_sequence_host *seq, double gamma, int64_t *n_events,
+                         double *flip_deg, int64_t capacity) {
+    return hom::guarded_host([&] {
+        return hom_kspace_rf_events_impl(seq, gamma, n_events, flip_deg, capacity);
+    });
 }
 
 }  // extern "C"

@@ -23,6 +23,7 @@
 #include "mixed_common.h"
 
 #include <cstring>
+#include <system_error>
 #include <thread>
 
 #ifndef HOM_GPT
@@ -358,8 +359,13 @@ double parallel_max(int64_t n, F &&fill) {
     }
     std::vector<std::thread> pool;
     pool.reserve((size_t)nth - 1);
-    for (int64_t i = 1; i < nth; ++i)
-        pool.emplace_back([&, i] { fill(n * i / nth, n * (i + 1) / nth, m[(size_t)i]); });
+    for (int64_t i = 1; i < nth; ++i) {
+        try {
+            pool.emplace_back([&, i] { fill(n * i / nth, n * (i + 1) / nth, m[(size_t)i]); });
+        } catch (const std::system_error &) {
+            fill(n * i / nth, n * (i + 1) / nth, m[(size_t)i]);  // no thread available: run it here
+        }
+    }
     fill(0, n / nth, m[0]);
     for (auto &th : pool) th.join();
     return *std::max_element(m.begin(), m.end());

@@ -1055,6 +1055,55 @@ int step_once(hom_server *s, int *worked) {
     return HOM_OK;
 }
 
+// step_once behind a catch: a C++ exception in a step (a failed host
+// allocation) stops the server like a device error: the active requests fail
+// and waiters wake.
+int step_guarded(hom_server *s, int *worked) {
+    std::string msg;
+    int code = HOM_ERR_STATE;
+    try {
+        return step_once(s, worked);
+    } catch (const std::bad_alloc &) {
+        msg = "The server stopped: host memory allocation failed in a step.";
+        code = HOM_ERR_MEMORY;
+    } catch (const std::exception &e) {
+        msg = std::string("The server stopped: ") + e.what();
+    } catch (...) {
+        msg = "The server stopped after an internal error.";
+    }
+    *worked = 0;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->fatal = true;
+    s->err = msg;
+    fail_active_locked(s, msg);
+    s->cv_done.notify_all();
+    return code;
+}
+
+// Runs f() for a server entry point (s is NULL while a server is created):
+// a failed host allocation becomes HOM_ERR_MEMORY, any other exception
+// HOM_ERR_STATE, so no C++ exception crosses the C ABI.
+template <typename F>
+int guarded_server(hom_server *s, F &&f) {
+    std::string msg;
+    int code = HOM_ERR_STATE;
+    try {
+        return f();
+    } catch (const std::bad_alloc &) {
+        msg = "Host memory allocation failed.";
+        code = HOM_ERR_MEMORY;
+    } catch (const std::exception &e) {
+        msg = std::string("Internal error: ") + e.what();
+    } catch (...) {
+        msg = "Internal error.";
+    }
+    if (!s) {
+        g_server_create_error = msg;
+        return code;
+    }
+    return fail(s, code, msg);
+}
+
 void worker_main(hom_server *s) {
     hipSetDevice(s->cfg.device);
     uint64_t blocked_at = ~(uint64_t)0;  // epoch at which the queue head could not be admitted
@@ -1077,7 +1126,7 @@ void worker_main(hom_server *s) {
             e0 = s->epoch;
         }
         int worked = 0;
-        step_once(s, &worked);
+        step_guarded(s, &worked);
         blocked_at = worked ? ~(uint64_t)0 : e0;
         {
             std::lock_guard<std::mutex> lk(s->mu);
@@ -1106,7 +1155,7 @@ const char *hom_server_last_error(const hom_server *s) {
     return s->err.c_str();
 }
 
-int hom_server_create(hom_server **out, const hom_server_config *config) {
+static int hom_server_create_impl(hom_server **out, const hom_server_config *config) {
     g_server_create_error.clear();
     if (!out) {
         g_server_create_error = "The output pointer is NULL.";
@@ -1210,9 +1259,9 @@ int hom_server_get_config(hom_server *s, hom_server_config *out) {
     return HOM_OK;
 }
 
-int hom_server_put_phantom(hom_server *s, int64_t ns, const double *x, const double *y,
-                           const double *z, const double *rho, const double *T1, const double *T2,
-                           const double *dw, int64_t *id) {
+static int hom_server_put_phantom_impl(hom_server *s, int64_t ns, const double *x, const double *y,
+                                       const double *z, const double *rho, const double *T1, const double *T2,
+                                       const double *dw, int64_t *id) {
     if (!s) return HOM_ERR_ARGUMENT;
     if (!id) return fail(s, HOM_ERR_ARGUMENT, "The id pointer is NULL.");
     if (ns < 1) return fail(s, HOM_ERR_DIMENSION, "A phantom needs at least one spin.");
@@ -1246,8 +1295,8 @@ int hom_server_put_phantom(hom_server *s, int64_t ns, const double *x, const dou
     return HOM_OK;
 }
 
-int hom_server_put_csm(hom_server *s, int64_t phantom, int64_t nc, const double *csm, int64_t ld,
-                       int64_t *id) {
+static int hom_server_put_csm_impl(hom_server *s, int64_t phantom, int64_t nc, const double *csm, int64_t ld,
+                                   int64_t *id) {
     if (!s) return HOM_ERR_ARGUMENT;
     if (!id || !csm) return fail(s, HOM_ERR_ARGUMENT, "A CSM pointer is NULL.");
     int64_t ns = 0;
@@ -1283,7 +1332,7 @@ int hom_server_put_csm(hom_server *s, int64_t phantom, int64_t nc, const double 
     return HOM_OK;
 }
 
-int hom_server_put_sequence(hom_server *s, const hom_sequence_host *seq, double gamma, int64_t *id) {
+static int hom_server_put_sequence_impl(hom_server *s, const hom_sequence_host *seq, double gamma, int64_t *id) {
     if (!s) return HOM_ERR_ARGUMENT;
     if (!id || !seq) return fail(s, HOM_ERR_ARGUMENT, "A sequence pointer is NULL.");
     const int64_t n = seq->n_nodes, ni = n - 1;
@@ -1338,7 +1387,7 @@ int hom_server_put_sequence(hom_server *s, const hom_sequence_host *seq, double 
     return HOM_OK;
 }
 
-int hom_server_release(hom_server *s, int kind, int64_t id) {
+static int hom_server_release_impl(hom_server *s, int kind, int64_t id) {
     if (!s) return HOM_ERR_ARGUMENT;
     hipSetDevice(s->cfg.device);
     std::lock_guard<std::mutex> lk(s->mu);
@@ -1368,7 +1417,7 @@ int hom_server_release(hom_server *s, int kind, int64_t id) {
     }
 }
 
-int hom_server_footprint(hom_server *s, const hom_request *req, int64_t *bytes) {
+static int hom_server_footprint_impl(hom_server *s, const hom_request *req, int64_t *bytes) {
     if (!s) return HOM_ERR_ARGUMENT;
     if (!req || !bytes) return fail(s, HOM_ERR_ARGUMENT, "A pointer is NULL.");
     Job j;
@@ -1387,7 +1436,7 @@ int hom_server_footprint(hom_server *s, const hom_request *req, int64_t *bytes) 
     return HOM_OK;
 }
 
-int hom_server_submit(hom_server *s, const hom_request *req, int64_t *rid) {
+static int hom_server_submit_impl(hom_server *s, const hom_request *req, int64_t *rid) {
     if (!s) return HOM_ERR_ARGUMENT;
     if (!req || !rid) return fail(s, HOM_ERR_ARGUMENT, "A pointer is NULL.");
     const double t = now_seconds();
@@ -1421,17 +1470,17 @@ int hom_server_submit(hom_server *s, const hom_request *req, int64_t *rid) {
     return HOM_OK;
 }
 
-int hom_server_step(hom_server *s, int *worked) {
+static int hom_server_step_impl(hom_server *s, int *worked) {
     if (!s) return HOM_ERR_ARGUMENT;
     if (s->cfg.threaded) return fail(s, HOM_ERR_STATE, "The worker thread runs the steps of this server.");
     int w = 0;
     std::lock_guard<std::mutex> step(s->step_mu);
-    const int rc = step_once(s, &w);
+    const int rc = step_guarded(s, &w);
     if (worked) *worked = w;
     return rc;
 }
 
-int hom_server_drain(hom_server *s) {
+static int hom_server_drain_impl(hom_server *s) {
     if (!s) return HOM_ERR_ARGUMENT;
     if (!s->cfg.threaded) {
         for (;;) {
@@ -1511,7 +1560,7 @@ int hom_server_result_shape(hom_server *s, int64_t rid, int64_t *rows, int64_t *
     return HOM_OK;
 }
 
-int hom_server_result(hom_server *s, int64_t rid, double *out, int64_t ld) {
+static int hom_server_result_impl(hom_server *s, int64_t rid, double *out, int64_t ld) {
     if (!s) return HOM_ERR_ARGUMENT;
     const double2 *src = nullptr;
     int64_t rows = 0, nc = 0;
@@ -1559,7 +1608,7 @@ int hom_server_times(hom_server *s, int64_t rid, double *t_submit, double *t_adm
     return HOM_OK;
 }
 
-int hom_server_get_stats(hom_server *s, hom_server_stats *out) {
+static int hom_server_get_stats_impl(hom_server *s, hom_server_stats *out) {
     if (!s || !out) return HOM_ERR_ARGUMENT;
     std::lock_guard<std::mutex> lk(s->mu);
     *out = s->stats;
@@ -1570,9 +1619,9 @@ int hom_server_get_stats(hom_server *s, hom_server_stats *out) {
     return HOM_OK;
 }
 
-int hom_server_autotune(hom_server *s, const hom_request *probe, const int *gpt_list, int n_gpt,
-                        const int *grid_list, int n_grid, int *best_gpt, int *best_grid,
-                        double *best_seconds) {
+static int hom_server_autotune_impl(hom_server *s, const hom_request *probe, const int *gpt_list, int n_gpt,
+                                    const int *grid_list, int n_grid, int *best_gpt, int *best_grid,
+                                    double *best_seconds) {
     if (!s) return HOM_ERR_ARGUMENT;
     if (!probe) return fail(s, HOM_ERR_ARGUMENT, "The probe request is NULL.");
     static const int kGpt[] = {64, 128, 256, 512};
@@ -1684,6 +1733,85 @@ int hom_server_autotune(hom_server *s, const hom_request *probe, const int *gpt_
     if (best_grid) *best_grid = bgrid;
     if (best_seconds) *best_seconds = best * 1e-3;
     return HOM_OK;
+}
+
+// Entry points that allocate host memory or start threads: no exception
+// crosses the C ABI.
+int hom_server_put_phantom(hom_server *s, int64_t ns, const double *x, const double *y,
+                           const double *z, const double *rho, const double *T1, const double *T2,
+                           const double *dw, int64_t *id) {
+    return guarded_server(s, [&] {
+        return hom_server_put_phantom_impl(s, ns, x, y, z, rho, T1, T2, dw, id);
+    });
+}
+
+int hom_server_put_csm(hom_server *s, int64_t phantom, int64_t nc, const double *csm, int64_t ld,
+                       int64_t *id) {
+    return guarded_server(s, [&] {
+        return hom_server_put_csm_impl(s, phantom, nc, csm, ld, id);
+    });
+}
+
+int hom_server_put_sequence(hom_server *s, const hom_sequence_host *seq, double gamma, int64_t *id) {
+    return guarded_server(s, [&] {
+        return hom_server_put_sequence_impl(s, seq, gamma, id);
+    });
+}
+
+int hom_server_release(hom_server *s, int kind, int64_t id) {
+    return guarded_server(s, [&] {
+        return hom_server_release_impl(s, kind, id);
+    });
+}
+
+int hom_server_footprint(hom_server *s, const hom_request *req, int64_t *bytes) {
+    return guarded_server(s, [&] {
+        return hom_server_footprint_impl(s, req, bytes);
+    });
+}
+
+int hom_server_submit(hom_server *s, const hom_request *req, int64_t *rid) {
+    return guarded_server(s, [&] {
+        return hom_server_submit_impl(s, req, rid);
+    });
+}
+
+int hom_server_step(hom_server *s, int *worked) {
+    return guarded_server(s, [&] {
+        return hom_server_step_impl(s, worked);
+    });
+}
+
+int hom_server_drain(hom_server *s) {
+    return guarded_server(s, [&] {
+        return hom_server_drain_impl(s);
+    });
+}
+
+int hom_server_result(hom_server *s, int64_t rid, double *out, int64_t ld) {
+    return guarded_server(s, [&] {
+        return hom_server_result_impl(s, rid, out, ld);
+    });
+}
+
+int hom_server_get_stats(hom_server *s, hom_server_stats *out) {
+    return guarded_server(s, [&] {
+        return hom_server_get_stats_impl(s, out);
+    });
+}
+
+int hom_server_autotune(hom_server *s, const hom_request *probe, const int *gpt_list, int n_gpt,
+                        const int *grid_list, int n_grid, int *best_gpt, int *best_grid,
+                        double *best_seconds) {
+    return guarded_server(s, [&] {
+        return hom_server_autotune_impl(s, probe, gpt_list, n_gpt, grid_list, n_grid, best_gpt, best_grid, best_seconds);
+    });
+}
+
+int hom_server_create(hom_server **out, const hom_server_config *config) {
+    return guarded_server(nullptr, [&] {
+        return hom_server_create_impl(out, config);
+    });
 }
 
 }  // extern "C"
