@@ -15,7 +15,9 @@
  * One plan is bound to one device and one HIP stream and owns every device
  * allocation it makes.  Calls on one plan must be serialized by the caller;
  * different plans may be used from different threads.  Shapes are validated
- * before any allocation or launch.
+ * before any allocation or launch.  No C++ exception leaves the library: a
+ * failed host allocation returns HOM_ERR_MEMORY, any other internal failure
+ * HOM_ERR_STATE, each with its text in hom_last_error().
  */
 #ifndef HOM_ABI_H
 #define HOM_ABI_H
