@@ -211,28 +211,42 @@ __global__ __launch_bounds__(256) void k_coil_partials(
         }
         __syncthreads();
     }
+    // channel-major partials: the tiles of one (sample, coil) channel are contiguous
+    const int64_t ntiles = gridDim.x;
     for (int j = 0; j < 4; ++j) {
         const int64_t k = k0 + kk, c = c0 + cq + 8 * j;
-        if (k < nadc && c < nc) partials[tile * nch + (k + c * nadc)] = acc[j];
+        if (k < nadc && c < nc) partials[(k + c * nadc) * ntiles + tile] = acc[j];
     }
 }
 
-// Stage 2: each lane sums one (sample, coil) channel over the spin tiles in
-// tile order and writes it to the (possibly strided) output.
+// Stage 2, as the reference's finalize (CoilReductionGPU.jl:44-69): one
+// workgroup per (sample, coil) channel; lane l sums tiles l, l + 256, ... and
+// a fixed tree in LDS combines the lanes, then the channel is written to the
+// (possibly strided) output.
 template <typename T>
-__global__ __launch_bounds__(256) void k_coil_finalize(int64_t ntiles, int64_t nadc, int64_t nch,
+__global__ __launch_bounds__(256) void k_coil_finalize(int64_t ntiles, int64_t nadc,
                                                       const Cx<T> *__restrict__ partials,
                                                       Cx<T> *__restrict__ out, int64_t ldo) {
-    const int64_t ch = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (ch >= nch) return;
+    __shared__ Cx<T> red[256];
+    const int64_t ch = blockIdx.x;
+    const int lane = threadIdx.x;
+    const Cx<T> *row = partials + ch * ntiles;
     Cx<T> v = {};
-    for (int64_t t = 0; t < ntiles; ++t) {
-        const Cx<T> p = partials[t * nch + ch];
-        v.re += p.re;
-        v.im += p.im;
+    for (int64_t t = lane; t < ntiles; t += 256) {
+        const Cx<T> q = row[t];
+        v.re += q.re;
+        v.im += q.im;
     }
-    const int64_t k = ch % nadc, c = ch / nadc;
-    out[k + c * ldo] = v;
+    red[lane] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (lane < off) {
+            red[lane].re += red[lane + off].re;
+            red[lane].im += red[lane + off].im;
+        }
+        __syncthreads();
+    }
+    if (lane == 0) out[ch % nadc + (ch / nadc) * ldo] = red[0];
 }
 
 template <typename T>
@@ -335,8 +349,8 @@ hipError_t reduce_t(hom_plan *p, const void *states, int64_t lds, int64_t nadc, 
     if (e != hipSuccess) return e;
     {
         ScopedTimer timer(p, HOM_K_COIL_FINALIZE);
-        hipLaunchKernelGGL((k_coil_finalize<T>), dim3(grid_for(nch, 256)), dim3(256), 0, p->stream,
-                           ntiles, nadc, nch, (const Cx<T> *)p->d.partials, (Cx<T> *)out, ldo);
+        hipLaunchKernelGGL((k_coil_finalize<T>), dim3((unsigned)nch), dim3(256), 0, p->stream,
+                           ntiles, nadc, (const Cx<T> *)p->d.partials, (Cx<T> *)out, ldo);
     }
     return hipGetLastError();
 }
