@@ -405,7 +405,7 @@ __device__ inline double wave_max(double v) {
 // out[j] (j = 1..16): max over spins of |basis term j|, evaluated as on the
 // host (no contraction), so the mixed path picks the same operand exponents.
 template <typename T>
-__global__ void k_basis_absmax(int64_t ns, const T *__restrict__ x, const T *__restrict__ y,
+__global__ __launch_bounds__(256) void k_basis_absmax(int64_t ns, const T *__restrict__ x, const T *__restrict__ y,
                                const T *__restrict__ z, const T *__restrict__ dw,
                                unsigned long long *__restrict__ out) {
 #pragma clang fp contract(off)
@@ -418,10 +418,20 @@ __global__ void k_basis_absmax(int64_t ns, const T *__restrict__ x, const T *__r
 #pragma unroll
         for (int j = 0; j < 16; ++j) m[j] = fmax(m[j], fabs(basis_term(j + 1, xs, ys, zs, ws)));
     }
+    // waves of the block, then one atomic per term and block (atomics on the
+    // same 16 words serialize, so they are kept few)
+    __shared__ double part[4][16];
+    const int w = threadIdx.x >> 6;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         const double v = wave_max(m[j]);
-        if ((threadIdx.x & 63) == 0) atomic_max_nonneg(out + j + 1, v);
+        if ((threadIdx.x & 63) == 0) part[w][j] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        const int j = threadIdx.x;
+        const double v = fmax(fmax(part[0][j], part[1][j]), fmax(part[2][j], part[3][j]));
+        atomic_max_nonneg(out + j + 1, v);
     }
 }
 
@@ -441,7 +451,7 @@ __global__ void k_csm_colmax2(int64_t ns, const Cx<T> *__restrict__ csm, unsigne
 }
 
 hipError_t launch_basis_absmax(hom_plan *p, unsigned long long *out) {
-    const unsigned grid = (unsigned)std::min<int64_t>(grid_for(p->ns, 256), 1024);
+    const unsigned grid = (unsigned)std::min<int64_t>(grid_for(p->ns, 256), 256);
     if (p->is_f32())
         hipLaunchKernelGGL((k_basis_absmax<float>), dim3(grid), dim3(256), 0, p->stream, p->ns,
                            (const float *)p->d.x, (const float *)p->d.y, (const float *)p->d.z,
