@@ -172,9 +172,14 @@ __global__ __launch_bounds__(256) void k_plain_excitation(
 }
 
 // Deterministic coil reduction, stage 1: one workgroup per (spin tile of
-// kSpinTile spins, 32-sample x 32-coil channel tile).  Each lane owns one
+// kSpinTile<T> spins, 32-sample x 32-coil channel tile).  Each lane owns one
 // sample and four coils of the tile; spins stream through LDS 32 at a time.
-constexpr int kSpinTile = 2048;
+// Spins per stage-1 workgroup: enough workgroups to keep the CUs busy (a
+// 1M-spin phantom gives 1024 FP64 or 2048 FP32 tiles; measured best of
+// 512/1024/2048 for each type).
+template <typename T>
+constexpr int64_t kSpinTile = sizeof(T) == 8 ? 1024 : 512;
+constexpr int64_t kSpinTileMin = 512;
 constexpr int kChunk = 32;
 
 template <typename T>
@@ -189,8 +194,8 @@ __global__ __launch_bounds__(256) void k_coil_partials(
     const int64_t k0 = (blockIdx.y % ktiles) * 32, c0 = (blockIdx.y / ktiles) * 32;
     const int kk = tid % 32, cq = tid / 32;  // sample kk, coils cq + 8j
     Cx<T> acc[4] = {};
-    const int64_t s_begin = tile * kSpinTile;
-    const int64_t s_end = std::min<int64_t>(ns, s_begin + kSpinTile);
+    const int64_t s_begin = tile * kSpinTile<T>;
+    const int64_t s_end = std::min<int64_t>(ns, s_begin + kSpinTile<T>);
     for (int64_t sc = s_begin; sc < s_end; sc += kChunk) {
         for (int j = 0; j < 4; ++j) {
             const int row = tid % 32, col = tid / 32 + 8 * j;
@@ -335,7 +340,7 @@ hipError_t reduce_t(hom_plan *p, const void *states, int64_t lds, int64_t nadc, 
                            p->stream, (Cx<T> *)out, nadc, nc, ldo);
         return hipGetLastError();
     }
-    const int64_t ntiles = (p->ns + kSpinTile - 1) / kSpinTile;
+    const int64_t ntiles = (p->ns + kSpinTile<T> - 1) / kSpinTile<T>;
     const int64_t nch = nadc * nc;
     const int64_t ktiles = (nadc + 31) / 32, ctiles = (nc + 31) / 32;
     {
@@ -357,8 +362,9 @@ hipError_t reduce_t(hom_plan *p, const void *states, int64_t lds, int64_t nadc, 
 
 }  // namespace
 
+// Partial elements of one reduction, for the smaller of the two spin tiles.
 int64_t coil_partial_elements(int64_t ns, int64_t n_adc, int64_t nc) {
-    const int64_t ntiles = (ns + kSpinTile - 1) / kSpinTile;
+    const int64_t ntiles = (ns + kSpinTileMin - 1) / kSpinTileMin;
     return std::max<int64_t>(ntiles, 1) * std::max<int64_t>(n_adc, 1) * std::max<int64_t>(nc, 1);
 }
 
