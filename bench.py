@@ -290,6 +290,16 @@ def main():
                      np.linalg.norm(ref[i * n:(i + 1) * n])) for i in range(cfg.n_interleaves)]
         accuracy = {"nrmse_vs_f64_oracle": float(np.linalg.norm(sig - ref) / np.linalg.norm(ref)),
                     "nrmse_max_per_interleave": max(per), "spins_checked": len(sub)}
+        # the plain drop-ins on the same spins
+        for prec in ("f64", "f32"):
+            with Plan(prec, len(sub), nc, seq.n_nodes, max_adc) as pd:
+                pd.upload_phantom(sub)
+                pd.upload_sequence(seq)
+                pd.run(parts, method)
+                sp = pd.download_signal()
+            accuracy[f"plain_{prec}_nrmse_vs_f64_oracle"] = float(np.linalg.norm(sp - ref) / np.linalg.norm(ref))
+        log(f"plain drop-in NRMSE vs f64 oracle: f64 {accuracy['plain_f64_nrmse_vs_f64_oracle']:.3e}, "
+            f"f32 {accuracy['plain_f32_nrmse_vs_f64_oracle']:.3e}")
         log(f"NRMSE vs f64 oracle: {accuracy['nrmse_vs_f64_oracle']:.3e} "
             f"(max per interleave {max(per):.3e})")
 
