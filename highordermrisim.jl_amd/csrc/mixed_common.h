@@ -473,8 +473,11 @@ __device__ inline StageSrc stage_src(int c, int nbop, int naho, int nrec, const 
     return {reinterpret_cast<const unsigned char *>(seed_b) + (int64_t)c * 16, 16 * 8};
 }
 
-// LDS layout of the split-phase body: two group records (double buffer) and
-// one private offset-operand slice per wave.
+// LDS of the split-phase body: two group-record buffers (double buffer) and
+// one private offset-operand slice per wave.  The three are separate
+// __shared__ arrays so the compiler can tell that the LDS reads of one record
+// buffer do not alias the direct global-to-LDS copy into the other, and does
+// not wait for that copy before reading.
 template <int NMH, int NCB>
 struct SplitLds {
     static constexpr int kBopChunks = NCB * 2 * 64;    // 16-byte chunks
@@ -482,10 +485,25 @@ struct SplitLds {
     static constexpr int kRecChunks = kBopChunks + kAhoChunks + 32 + 4 + 4 + 8;
     static constexpr int kRecBytes = kRecChunks * 16;
     static constexpr int kBvBytes = kTiles * (NMH > 0 ? NMH : 1) * 64 * 16;  // per wave
-    static constexpr int kBufs = 2;  // group-record buffers (double buffer)
-    static constexpr int kBytes = kBufs * kRecBytes + kWaves * kBvBytes;
     static constexpr int kStage = (kRecChunks + 255) / 256;  // chunks per thread
 };
+
+struct SplitSmem {
+    unsigned char *rec0, *rec1;  // group records, SplitLds::kRecBytes each
+    unsigned char *bv;           // kWaves slices of SplitLds::kBvBytes
+};
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) const void global_cvoid_t;
+
+// Direct 16-byte global-to-LDS copy (global_load_lds_dwordx4): lane l of the
+// wave writes lds_wave + 16 l; lds_wave must be the same for the whole wave.
+__device__ inline void copy16_to_lds(const unsigned char *src, unsigned char *lds_wave) {
+    __builtin_amdgcn_global_load_lds((global_cvoid_t *)src, (lds_void_t *)lds_wave, 16, 0, 0);
+}
+
+// Wait until this wave's global loads, and with them its LDS copies, are done.
+__device__ inline void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // vmcnt(0)
 
 // Split-phase body (FP64 first order + FP16 higher-order offsets) for one
 // work item.  Workgroup = kWaves waves over one spin tile; wave w owns kTiles
@@ -495,14 +513,15 @@ struct SplitLds {
 // The spin-group operands (coil B operand, higher-order A operand, FP64
 // basis, decay rate, log-magnitude, phase seed) are the same for all waves:
 // the workgroup copies each group's record into LDS once, double-buffered,
-// while it computes the previous group (global loads into staging registers
-// before the compute, LDS stores after it, one barrier per group).  Each wave
-// keeps its own per-tile FP16 offset operands in a private LDS slice.  Every
-// wave takes part in the copies and barriers, with or without tiles; the
-// body ends on a barrier, so a persistent caller may reuse smem at once.
+// with direct global-to-LDS loads issued before the compute of the previous
+// group and awaited after it, one barrier per group.  Groups go in pairs, the
+// even one from rec0 and the odd one from rec1.  Each wave keeps its own
+// per-tile FP16 offset operands in a private LDS slice.  Every wave takes
+// part in the copies and barriers, with or without tiles; the body ends on a
+// barrier, so a persistent caller may reuse the LDS at once.
 template <int NMH, int NCB>
 __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem &it,
-                                                unsigned char *smem) {
+                                                const SplitSmem &sm) {
     using L = SplitLds<NMH, NCB>;
     // Opaque thread id: a persistent caller runs this body in a loop, and
     // lane-dependent addresses must not be hoisted out of it (they would stay
@@ -513,7 +532,7 @@ __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem
     const int w = tid >> 6;
     const bool active = it.valid;
 
-    uint4 *bvl = reinterpret_cast<uint4 *>(smem + L::kBufs * L::kRecBytes + w * L::kBvBytes);
+    uint4 *bvl = reinterpret_cast<uint4 *>(sm.bv + w * L::kBvBytes);
     double kv[kTiles];
     float tkv[kTiles];
 #pragma unroll
@@ -529,27 +548,19 @@ __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem
     }
 
     // Each thread copies chunks tid, tid + 256 and tid + 512 of every group
-    // record.  The three slots are separate named registers: an array indexed
-    // in a loop is placed in LDS by the compiler (before unrolling makes the
-    // index constant), which costs 48 bytes of LDS per thread.
+    // record; wave w's lanes write the contiguous chunks 64 w + 256 k + lane.
     static_assert(L::kStage <= 3, "a group record has at most 768 chunks");
     const double *seed_b = o.seed + it.anchor * o.ngroups * 16;
     const StageSrc s0 = stage_src<NMH>(tid, L::kBopChunks, L::kAhoChunks, L::kRecChunks, o, seed_b);
     const StageSrc s1 = stage_src<NMH>(tid + 256, L::kBopChunks, L::kAhoChunks, L::kRecChunks, o, seed_b);
     const StageSrc s2 = stage_src<NMH>(tid + 512, L::kBopChunks, L::kAhoChunks, L::kRecChunks, o, seed_b);
     const bool u0 = tid < L::kRecChunks, u1 = tid + 256 < L::kRecChunks, u2 = tid + 512 < L::kRecChunks;
-    uint4 st0 = uint4{0, 0, 0, 0}, st1 = st0, st2 = st0;
-#define HOM_LOAD_STAGE(GRP)                                                          \
-    if (u0) st0 = *reinterpret_cast<const uint4 *>(s0.p + (GRP) * s0.stride);        \
-    if (L::kStage > 1 && u1) st1 = *reinterpret_cast<const uint4 *>(s1.p + (GRP) * s1.stride); \
-    if (L::kStage > 2 && u2) st2 = *reinterpret_cast<const uint4 *>(s2.p + (GRP) * s2.stride);
-#define HOM_STORE_STAGE(BUF)                                                         \
-    {                                                                                \
-        uint4 *dst_ = reinterpret_cast<uint4 *>(smem + (BUF) * L::kRecBytes);        \
-        if (u0) dst_[tid] = st0;                                                     \
-        if (L::kStage > 1 && u1) dst_[tid + 256] = st1;                              \
-        if (L::kStage > 2 && u2) dst_[tid + 512] = st2;                              \
-    }
+    const int wbase = (tid & ~63) * 16;
+    auto stage = [&](int64_t grp, unsigned char *dst) {
+        if (u0) copy16_to_lds(s0.p + grp * s0.stride, dst + wbase);
+        if (L::kStage > 1 && u1) copy16_to_lds(s1.p + grp * s1.stride, dst + wbase + 256 * 16);
+        if (L::kStage > 2 && u2) copy16_to_lds(s2.p + grp * s2.stride, dst + wbase + 512 * 16);
+    };
 
     floatx4 acc[kTiles][NCB];
 #pragma unroll
@@ -557,77 +568,78 @@ __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem
 #pragma unroll
         for (int c = 0; c < NCB; ++c) acc[t][c] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-    const int64_t g0 = it.stile * o.gpt;
-    const int64_t g1 = std::min<int64_t>(g0 + o.gpt, o.ngroups);
-    HOM_LOAD_STAGE(g0)
-    HOM_STORE_STAGE(0)
-    __syncthreads();
-    for (int64_t grp = g0; grp < g1; ++grp) {
-        const int cur = (int)((grp - g0) % L::kBufs);
-        const int nxt = (cur + 1) % L::kBufs;
-        const bool more = grp + 1 < g1;
-        if (more) {
-            HOM_LOAD_STAGE(grp + 1)
+    auto compute = [&](const unsigned char *rec) {
+        const half8_t *rb = reinterpret_cast<const half8_t *>(rec);
+        half8_t bh[NCB], bl[NCB];
+#pragma unroll
+        for (int c = 0; c < NCB; ++c) {
+            bh[c] = rb[(c * 2 + 0) * 64 + lane];
+            bl[c] = rb[(c * 2 + 1) * 64 + lane];
         }
-        if (active) {
-            const unsigned char *rec = smem + cur * L::kRecBytes;
-            const half8_t *rb = reinterpret_cast<const half8_t *>(rec);
-            half8_t bh[NCB], bl[NCB];
+        half8_t ah[NMH > 0 ? NMH : 1];
+#pragma unroll
+        for (int m = 0; m < NMH; ++m) ah[m] = rb[L::kBopChunks + m * 64 + lane];
+        const unsigned char *tail = rec + (L::kBopChunks + L::kAhoChunks) * 16;
+        const double ba = reinterpret_cast<const double *>(tail)[lane];
+        const floatx4 cd = reinterpret_cast<const floatx4 *>(tail + 512)[lane >> 4];
+        const floatx4 lmv = reinterpret_cast<const floatx4 *>(tail + 576)[lane >> 4];
+        // Two 16-byte reads: a 32-byte LDS read would make the compiler wait
+        // for the in-flight copy into the other record buffer.
+        const double2 sd01 = reinterpret_cast<const double2 *>(tail + 640)[2 * (lane >> 4)];
+        const double2 sd23 = reinterpret_cast<const double2 *>(tail + 640)[2 * (lane >> 4) + 1];
+        const doublex4 sd = {sd01.x, sd01.y, sd23.x, sd23.y};
+        const half8_t *bv = reinterpret_cast<const half8_t *>(bvl);
+#pragma unroll
+        for (int t = 0; t < kTiles; ++t) {
+            const doublex4 ph = __builtin_amdgcn_mfma_f64_16x16x4f64(ba, kv[t], sd, 0, 0, 0);
+            floatx4 off = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int m = 0; m < NMH; ++m)
+                off = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[m], bv[(t * NMH + m) * 64 + lane],
+                                                             off, 0, 0, 0);
+            uint4 ahu, alu;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                float rf = reduce_rev(ph[v]);
+                if (NMH > 0) rf = fmaf(off[v], 0x1p-20f, rf);
+                const StateSplit e = state_split(rf, cd[v], tkv[t], lmv[v]);
+                (&ahu.x)[v] = __builtin_bit_cast(unsigned int, e.hi);
+                (&alu.x)[v] = __builtin_bit_cast(unsigned int, e.lo);
+            }
+            const half8_t ahi = __builtin_bit_cast(half8_t, ahu);
+            const half8_t alo = __builtin_bit_cast(half8_t, alu);
 #pragma unroll
             for (int c = 0; c < NCB; ++c) {
-                bh[c] = rb[(c * 2 + 0) * 64 + lane];
-                bl[c] = rb[(c * 2 + 1) * 64 + lane];
+                acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bh[c], acc[t][c], 0, 0, 0);
+                acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bl[c], acc[t][c], 0, 0, 0);
+                acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo, bh[c], acc[t][c], 0, 0, 0);
             }
-            half8_t ah[NMH > 0 ? NMH : 1];
-#pragma unroll
-            for (int m = 0; m < NMH; ++m) ah[m] = rb[L::kBopChunks + m * 64 + lane];
-            const unsigned char *tail = rec + (L::kBopChunks + L::kAhoChunks) * 16;
-            const double ba = reinterpret_cast<const double *>(tail)[lane];
-            const floatx4 cd = reinterpret_cast<const floatx4 *>(tail + 512)[lane >> 4];
-            const floatx4 lmv = reinterpret_cast<const floatx4 *>(tail + 576)[lane >> 4];
-            const doublex4 sd = reinterpret_cast<const doublex4 *>(tail + 640)[lane >> 4];
-            const half8_t *bv = reinterpret_cast<const half8_t *>(bvl);
-#pragma unroll
-            for (int t = 0; t < kTiles; ++t) {
-                const doublex4 ph = __builtin_amdgcn_mfma_f64_16x16x4f64(ba, kv[t], sd, 0, 0, 0);
-                floatx4 off = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int m = 0; m < NMH; ++m)
-                    off = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[m], bv[(t * NMH + m) * 64 + lane],
-                                                                 off, 0, 0, 0);
-                uint4 ahu, alu;
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    float rf = reduce_rev(ph[v]);
-                    if (NMH > 0) rf = fmaf(off[v], 0x1p-20f, rf);
-                    const StateSplit e = state_split(rf, cd[v], tkv[t], lmv[v]);
-                    (&ahu.x)[v] = __builtin_bit_cast(unsigned int, e.hi);
-                    (&alu.x)[v] = __builtin_bit_cast(unsigned int, e.lo);
-                }
-                const half8_t ahi = __builtin_bit_cast(half8_t, ahu);
-                const half8_t alo = __builtin_bit_cast(half8_t, alu);
-#pragma unroll
-                for (int c = 0; c < NCB; ++c) {
-                    acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bh[c], acc[t][c], 0, 0, 0);
-                    acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bl[c], acc[t][c], 0, 0, 0);
-                    acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo, bh[c], acc[t][c], 0, 0, 0);
-                }
-            }
+        }
 #if HOM_SCHED_VALU > 0
 #pragma unroll
-            for (int i = 0; i < kTiles * (1 + NMH + 3 * NCB); ++i) {
-                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);               // one MFMA
-                __builtin_amdgcn_sched_group_barrier(0x002, HOM_SCHED_VALU, 0);  // then VALU
-            }
+        for (int i = 0; i < kTiles * (1 + NMH + 3 * NCB); ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);               // one MFMA
+            __builtin_amdgcn_sched_group_barrier(0x002, HOM_SCHED_VALU, 0);  // then VALU
+        }
 #endif
-        }
-        if (more) {
-            HOM_STORE_STAGE(nxt)
-        }
+    };
+
+    const int64_t g0 = it.stile * o.gpt;
+    const int64_t g1 = std::min<int64_t>(g0 + o.gpt, o.ngroups);
+    stage(g0, sm.rec0);
+    wait_vmem();
+    __syncthreads();
+    for (int64_t grp = g0; grp < g1; grp += 2) {
+        if (grp + 1 < g1) stage(grp + 1, sm.rec1);
+        if (active) compute(sm.rec0);
+        wait_vmem();
+        __syncthreads();
+        if (grp + 1 >= g1) break;
+        if (grp + 2 < g1) stage(grp + 2, sm.rec0);
+        if (active) compute(sm.rec1);
+        wait_vmem();
         __syncthreads();
     }
-#undef HOM_LOAD_STAGE
-#undef HOM_STORE_STAGE
     if (active) write_partials<NCB>(acc, it, o);
 }
 

@@ -179,10 +179,13 @@ __device__ inline int64_t xcd_work() {
 template <int NMH, int NCB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HOM_WAVES_PER_EU, HOM_WAVES_PER_EU))) void k_mix_main(
     MainOps o, int64_t nsb, int64_t n_work) {
-    __shared__ __attribute__((aligned(32))) unsigned char smem[SplitLds<NMH, NCB>::kBytes];
+    using L = SplitLds<NMH, NCB>;
+    __shared__ __attribute__((aligned(32))) unsigned char rec0[L::kRecBytes];
+    __shared__ __attribute__((aligned(32))) unsigned char rec1[L::kRecBytes];
+    __shared__ __attribute__((aligned(32))) unsigned char bv[kWaves * L::kBvBytes];
     const int64_t work = xcd_work();
     if (work >= n_work) return;  // whole workgroup
-    main_split_body<NMH, NCB>(o, make_item(o, work % nsb, work / nsb), smem);
+    main_split_body<NMH, NCB>(o, make_item(o, work % nsb, work / nsb), SplitSmem{rec0, rec1, bv});
 }
 
 template <int NI, int NCB>

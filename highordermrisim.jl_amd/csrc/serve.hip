@@ -170,11 +170,11 @@ __device__ inline int64_t uniform64(int64_t v) {
 
 // Body of one work item for each kernel class.
 __device__ __forceinline__ void run_class(int cls, const MainOps &o, int64_t sb, int64_t st,
-                                          unsigned char *smem) {
+                                          const SplitSmem &sm) {
     const WorkItem it = make_item(o, sb, st);
     switch (cls) {
 #define HOM_SPLIT(C, NMH, NCB) \
-    case C: main_split_body<NMH, NCB>(o, it, smem); break;
+    case C: main_split_body<NMH, NCB>(o, it, sm); break;
 #define HOM_FP64(C, NI, NCB) \
     case C: main64_body<NI, NCB>(o, it); break;
         HOM_SPLIT(0, 0, 1) HOM_SPLIT(1, 0, 2) HOM_SPLIT(2, 0, 3) HOM_SPLIT(3, 0, 4)
@@ -194,7 +194,12 @@ __device__ __forceinline__ void run_class(int cls, const MainOps &o, int64_t sb,
 // range is done; item -> entry by binary search over the entries' item_lo.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HOM_WAVES_PER_EU, HOM_WAVES_PER_EU))) void k_srv_fused(
     const StepDesc *__restrict__ sd, unsigned int *__restrict__ ctr) {
-    __shared__ __attribute__((aligned(32))) unsigned char smem[SplitLds<2, 4>::kBytes];
+    // LDS of the largest split class (NMH 2, NCB 4) serves every class.
+    using L = SplitLds<2, 4>;
+    __shared__ __attribute__((aligned(32))) unsigned char rec0[L::kRecBytes];
+    __shared__ __attribute__((aligned(32))) unsigned char rec1[L::kRecBytes];
+    __shared__ __attribute__((aligned(32))) unsigned char bv[kWaves * L::kBvBytes];
+    const SplitSmem sm{rec0, rec1, bv};
     __shared__ int64_t s_item;
     const int xcd = blockIdx.x % 8;
     const int64_t total = sd->total_items;
@@ -213,7 +218,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HOM_WAVES_P
         }
         const MainDesc &md = sd->main[b];
         const int64_t local = item - md.item_lo;
-        run_class(md.cls, md.o, local % md.nsb, local / md.nsb, smem);
+        run_class(md.cls, md.o, local % md.nsb, local / md.nsb, sm);
     }
 }
 
