@@ -171,57 +171,153 @@ __global__ __launch_bounds__(256) void k_plain_excitation(
     mz[s] = mzv;
 }
 
-// Deterministic coil reduction, stage 1: one workgroup per (spin tile of
-// kSpinTile<T> spins, 32-sample x 32-coil channel tile).  Each lane owns one
-// sample and four coils of the tile; spins stream through LDS 32 at a time.
-// Spins per stage-1 workgroup: enough workgroups to keep the CUs busy (a
-// 1M-spin phantom gives 1024 FP64 or 2048 FP32 tiles; measured best of
-// 512/1024/2048 for each type).
+// Deterministic coil reduction, stage 1: the complex GEMM statesᵀ·csm on
+// the matrix cores as a real GEMM: K runs over (spin, re|im), the A operand
+// is the state's real or imaginary part, the B operand (re, im; -im, re) of
+// the sensitivity, so the 16 output columns of one MFMA tile are 8 coils x
+// (re, im).  One workgroup per (spin share, 32-sample x 32-coil channel
+// tile); the spins go in 16-spin chunks, and workgroup g of G takes chunks
+// g, g + G, g + 2G, ... so that the workgroups in flight read neighbouring
+// runs of every state and sensitivity row (owning contiguous spin tiles
+// instead ran 8-19 % slower).  G = ceil(Ns / kSpinTile): 512 workgroups for
+// 1M spins, the fastest measured (of 256-2048 for FP64, 512-4096 for FP32).
+// Wave w
+// owns sample tile w & 1 and coils 16 (w >> 1) .. +15 (two 16x16
+// accumulators); chunks pass through LDS spin-major, rows padded by one
+// element (conflict-free transposed stores and fragment reads),
+// double-buffered, with the global loads two chunks ahead of the compute.
+// The partial of workgroup g sums its chunks in a fixed order: the result is
+// deterministic.
+constexpr int64_t kSpinTile = 2048;
+constexpr int64_t kSpinTileMin = kSpinTile;
+constexpr int kCoilChunk = 16;
+
 template <typename T>
-constexpr int64_t kSpinTile = sizeof(T) == 8 ? 1024 : 512;
-constexpr int64_t kSpinTileMin = 512;
-constexpr int kChunk = 32;
+struct CoilMfma;
+template <>
+struct CoilMfma<double> {
+    typedef double acc_t __attribute__((ext_vector_type(4)));
+    __device__ static acc_t mma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // output row of accumulator element v in lane l (column l % 16)
+    __device__ static int row(int l, int v) { return (l >> 4) + 4 * v; }
+};
+template <>
+struct CoilMfma<float> {
+    typedef float acc_t __attribute__((ext_vector_type(4)));
+    __device__ static acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static int row(int l, int v) { return 4 * (l >> 4) + v; }
+};
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_coil_partials(
     int64_t ns, int64_t nadc, int64_t nc, const Cx<T> *__restrict__ states, int64_t lds,
     const Cx<T> *__restrict__ csm, int64_t ldc, Cx<T> *__restrict__ partials, int64_t nch) {
-    __shared__ Cx<T> st[kChunk][kChunk + 1];
-    __shared__ Cx<T> cs[kChunk][kChunk + 1];
-    const int tid = threadIdx.x;
+    using M = CoilMfma<T>;
+    constexpr int kC = kCoilChunk;
+    __shared__ Cx<T> st[2][kC][33];  // [buffer][spin][sample]
+    __shared__ Cx<T> cs[2][kC][33];  // [buffer][spin][coil]
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int64_t tile = blockIdx.x;
     const int64_t ktiles = (nadc + 31) / 32;
     const int64_t k0 = (blockIdx.y % ktiles) * 32, c0 = (blockIdx.y / ktiles) * 32;
-    const int kk = tid % 32, cq = tid / 32;  // sample kk, coils cq + 8j
-    Cx<T> acc[4] = {};
-    const int64_t s_begin = tile * kSpinTile<T>;
-    const int64_t s_end = std::min<int64_t>(ns, s_begin + kSpinTile<T>);
-    for (int64_t sc = s_begin; sc < s_end; sc += kChunk) {
-        for (int j = 0; j < 4; ++j) {
-            const int row = tid % 32, col = tid / 32 + 8 * j;
-            const int64_t s = sc + row;
-            const int64_t k = k0 + col, c = c0 + col;
-            st[row][col] = (s < s_end && k < nadc) ? states[s + k * lds] : Cx<T>{};
-            cs[row][col] = (s < s_end && c < nc) ? csm[s + c * ldc] : Cx<T>{};
+    // chunk i of this workgroup: global chunk tile + i * gridDim.x
+    const int64_t nchunks = (ns + kC - 1) / kC, stride = gridDim.x;
+    const int64_t nck = (nchunks - 1 - tile) / stride + 1;  // tile < nchunks
+    auto chunk0 = [&](int64_t i) { return (tile + i * stride) * kC; };
+    const int64_t s_hi = ns;
+
+    // staging: element e = tid + 256 j of a chunk is spin e % kC, sample/coil e / kC;
+    // two register sets (chunks i + 1 and i + 2 in flight while chunk i is in use)
+    struct Stage {
+        Cx<T> s[2], c[2];
+    };
+    // Loads are unconditional, from clamped in-bounds addresses (ns, nadc and
+    // nc are at least 1 here), and the bounds apply as selects at the store:
+    // with no branch around them the compiler waits only for the older
+    // register set before a store, not for every load in flight.
+    auto load = [&](Stage &r, int64_t i) {
+        const int64_t sc = chunk0(i);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int e = tid + 256 * j, sl = e % kC, q = e / kC;
+            const int64_t sp = std::min<int64_t>(sc + sl, s_hi - 1);
+            r.s[j] = states[sp + std::min<int64_t>(k0 + q, nadc - 1) * lds];
+            r.c[j] = csm[sp + std::min<int64_t>(c0 + q, nc - 1) * ldc];
         }
-        __syncthreads();
-#pragma unroll 4
-        for (int r = 0; r < kChunk; ++r) {
-            const Cx<T> a = st[r][kk];
-            for (int j = 0; j < 4; ++j) {
-                const Cx<T> w = cs[r][cq + 8 * j];
-                acc[j].re += a.re * w.re - a.im * w.im;
-                acc[j].im += a.re * w.im + a.im * w.re;
+    };
+    auto store = [&](const Stage &r, int b, int64_t i) {
+        const int64_t sc = chunk0(i);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int e = tid + 256 * j, sl = e % kC, q = e / kC;
+            const bool in = i < nck && sc + sl < s_hi;
+            st[b][sl][q] = (in && k0 + q < nadc) ? r.s[j] : Cx<T>{};
+            cs[b][sl][q] = (in && c0 + q < nc) ? r.c[j] : Cx<T>{};
+        }
+    };
+
+    // fragment roles of this lane: K index kk = lane / 16 -> spin kk / 2 of the
+    // step, part kk % 2; A row (sample) and B column (coil, re|im) lane % 16
+    const int mt = w & 1, nh = w >> 1;
+    const int kk = lane >> 4, n = lane & 15;
+    const int part = kk & 1, q = n & 1;
+    const bool b_re = part == q;                       // (re, im; -im, re)
+    const T b_sign = (part == 1 && q == 0) ? T(-1) : T(1);
+    const int a_col = 16 * mt + n;
+    const int b_col0 = 16 * nh + (n >> 1);
+    typename M::acc_t acc[2] = {};
+    auto compute = [&](int b) {
+#pragma unroll
+        for (int ks = 0; ks < kC / 2; ++ks) {
+            const int sl = 2 * ks + (kk >> 1);
+            const Cx<T> av = st[b][sl][a_col];
+            const T a = part ? av.im : av.re;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const Cx<T> bv = cs[b][sl][b_col0 + 8 * j];
+                acc[j] = M::mma(a, (b_re ? bv.re : bv.im) * b_sign, acc[j]);
             }
         }
+    };
+
+    // chunks in pairs: even chunks in LDS buffer 0 via register set ra, odd
+    // chunks in buffer 1 via rb.  Stores are unconditional as well: one past
+    // the last chunk writes zeros into a buffer that is not read again.
+    Stage ra, rb;
+    load(ra, 0);
+    load(rb, 1);
+    store(ra, 0, 0);
+    __syncthreads();
+    // sched_barrier: keep each chunk's loads ahead of the compute (the
+    // scheduler otherwise sinks them to the end of it)
+    for (int64_t i = 0; i < nck; i += 2) {
+        load(ra, i + 2);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(0);
+        store(rb, 1, i + 1);
+        __syncthreads();
+        if (i + 1 >= nck) break;
+        load(rb, i + 3);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(1);
+        store(ra, 0, i + 2);
         __syncthreads();
     }
     // channel-major partials: the tiles of one (sample, coil) channel are contiguous
     const int64_t ntiles = gridDim.x;
-    for (int j = 0; j < 4; ++j) {
-        const int64_t k = k0 + kk, c = c0 + cq + 8 * j;
-        if (k < nadc && c < nc) partials[(k + c * nadc) * ntiles + tile] = acc[j];
-    }
+    T *out = reinterpret_cast<T *>(partials);
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int64_t k = k0 + 16 * mt + M::row(lane, v);
+            const int64_t c = c0 + 16 * nh + 8 * j + (n >> 1);
+            if (k < nadc && c < nc) out[((k + c * nadc) * ntiles + tile) * 2 + q] = acc[j][v];
+        }
 }
 
 // Stage 2, as the reference's finalize (CoilReductionGPU.jl:44-69): one
@@ -340,7 +436,7 @@ hipError_t reduce_t(hom_plan *p, const void *states, int64_t lds, int64_t nadc, 
                            p->stream, (Cx<T> *)out, nadc, nc, ldo);
         return hipGetLastError();
     }
-    const int64_t ntiles = (p->ns + kSpinTile<T> - 1) / kSpinTile<T>;
+    const int64_t ntiles = (p->ns + kSpinTile - 1) / kSpinTile;
     const int64_t nch = nadc * nc;
     const int64_t ktiles = (nadc + 31) / 32, ctiles = (nc + 31) / 32;
     {
