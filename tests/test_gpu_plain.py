@@ -190,6 +190,9 @@ def test_coil_reduction(dtype):
     for nspins in (0, 1, 259, 2049, 32769):
         for nadc, nc in ((0, 3), (1, 1), (15, 3), (32, 32)):
             check_reduction(dtype, nspins, nadc, nc)
+    # several sample and coil tiles over interleaved spin chunks
+    check_reduction(dtype, 4097, 40, 33)
+    check_reduction(dtype, 70001, 65, 47)
     check_reduction(np.complex64, 1, 257, 257)
 
 
