@@ -589,14 +589,26 @@ __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem
         const double2 sd23 = reinterpret_cast<const double2 *>(tail + 640)[2 * (lane >> 4) + 1];
         const doublex4 sd = {sd01.x, sd01.y, sd23.x, sd23.y};
         const half8_t *bv = reinterpret_cast<const half8_t *>(bvl);
+        // phase and offset MFMAs of every tile first, then states + coil GEMM
+        // per tile (measured 1.8 % faster than phase, states and GEMM tile by
+        // tile: the second tile's offsets no longer wait behind the first
+        // tile's coil MFMAs)
+        doublex4 pht[kTiles];
+        floatx4 offt[kTiles];
+#pragma unroll
+        for (int t = 0; t < kTiles; ++t) pht[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ba, kv[t], sd, 0, 0, 0);
 #pragma unroll
         for (int t = 0; t < kTiles; ++t) {
-            const doublex4 ph = __builtin_amdgcn_mfma_f64_16x16x4f64(ba, kv[t], sd, 0, 0, 0);
-            floatx4 off = {0.f, 0.f, 0.f, 0.f};
+            offt[t] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int m = 0; m < NMH; ++m)
-                off = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[m], bv[(t * NMH + m) * 64 + lane],
-                                                             off, 0, 0, 0);
+                offt[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[m], bv[(t * NMH + m) * 64 + lane],
+                                                                 offt[t], 0, 0, 0);
+        }
+#pragma unroll
+        for (int t = 0; t < kTiles; ++t) {
+            const doublex4 ph = pht[t];
+            const floatx4 off = offt[t];
             uint4 ahu, alu;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
