@@ -314,9 +314,17 @@ __device__ inline void seed_elem(int64_t gi, int64_t ns, int64_t ngroups, int64_
     for (int q = 0; q < 12; ++q)
         b[q] = (s < ns && q < ps.nho) ? basis_term(ps.codeho[q], x[s], y[s], z[s], dw[s]) : 0.0;
     const double t0 = th0[gi];
+    // Fixed trip count, unconditional loads from a clamped index (b[q] is 0
+    // for q >= nho, and fma(0, a, v) = v): b stays in registers and the
+    // anchors of one block are loaded together.  A runtime-bound loop indexed
+    // b indirectly and waited for each scalar load in turn.
+    const int qmax = ps.nho > 0 ? ps.nho - 1 : 0;
     for (int64_t ab = 0; ab < nab; ++ab) {
         double v = t0;
-        for (int q = 0; q < ps.nho; ++q) v += b[q] * anc[ab * ps.nho + q];
+        if (ps.nho > 0) {
+#pragma unroll
+            for (int q = 0; q < 12; ++q) v = fma(b[q], anc[ab * ps.nho + (q < qmax ? q : qmax)], v);
+        }
         seed[ab * ngroups * 16 + gi] = v;
     }
 }
@@ -330,6 +338,8 @@ __device__ inline void finalize_elem(int64_t idx, int64_t nc, int64_t nstiles, i
                                      int64_t ld) {
     const int64_t c = idx % nc, k = idx / nc;
     double re = 0.0, im = 0.0;
+    // unrolled so that eight loads are in flight; the sum stays in tile order
+#pragma unroll 8
     for (int64_t st = 0; st < nstiles; ++st) {
         const float2 p = *reinterpret_cast<const float2 *>(partials + (st * rows + k) * cols + 2 * c);
         re += p.x;
