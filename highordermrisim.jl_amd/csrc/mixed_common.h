@@ -266,15 +266,10 @@ __device__ inline void bop_elem(int64_t idx, int64_t ns, int64_t nc, int NCB, co
 
 // ---- per-segment spin-side steps ----
 
-// Block-wide max of |M0| over spins s = base + threadIdx.x (256 threads), one
-// atomic per workgroup (nonnegative floats order like their bit patterns).
-__device__ inline void m0max_block(int64_t s, int64_t ns, const double2 *mxy, float *out) {
+// Workgroup (256 threads) maximum of v >= 0 into *out by one atomic
+// (non-negative floats order like their bit patterns).
+__device__ inline void m0max_reduce(float v, float *out) {
     __shared__ float red[4];
-    float v = 0.0f;
-    if (s < ns) {
-        const double2 m = mxy[s];
-        v = (float)sqrt(m.x * m.x + m.y * m.y);
-    }
     for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
@@ -283,6 +278,16 @@ __device__ inline void m0max_block(int64_t s, int64_t ns, const double2 *mxy, fl
         if (m > 0.0f) atomicMax(reinterpret_cast<unsigned int *>(out), __float_as_uint(m));
     }
     __syncthreads();  // red is reused by the next call of a persistent block
+}
+
+// Block-wide max of |M0| over spins s = base + threadIdx.x.
+__device__ inline void m0max_block(int64_t s, int64_t ns, const double2 *mxy, float *out) {
+    float v = 0.0f;
+    if (s < ns) {
+        const double2 m = mxy[s];
+        v = (float)sqrt(m.x * m.x + m.y * m.y);
+    }
+    m0max_reduce(v, out);
 }
 
 // idx < ngroups * 16, MFMA output-row order: th0 = arg(M0) / 2pi and

@@ -145,9 +145,17 @@ __global__ void k_mix_bop(int64_t ns, int64_t nc, int64_t ngroups, int NCB,
     if (idx < ngroups * NCB * 64) bop_elem(idx, ns, nc, NCB, csm, ldc, csc, bop);
 }
 
+// Grid-stride over the spins with a capped grid: one atomic per workgroup
+// (one per 256 spins serialised 4096 atomics on one word at 1M spins).
 __global__ __launch_bounds__(256) void k_mix_m0max(int64_t ns, const double2 *__restrict__ mxy,
                                                    float *__restrict__ out) {
-    m0max_block((int64_t)blockIdx.x * blockDim.x + threadIdx.x, ns, mxy, out);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    float v = 0.0f;
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += stride) {
+        const double2 m = mxy[s];
+        v = fmaxf(v, (float)sqrt(m.x * m.x + m.y * m.y));
+    }
+    m0max_reduce(v, out);
 }
 
 __global__ void k_mix_seg(int64_t ns, int64_t ngroups, const double2 *__restrict__ mxy,
@@ -703,7 +711,8 @@ int mixed_run(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *
             {
                 ScopedTimer timer(p, HOM_K_MIXED_PREP);
                 hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, p->stream, W.m0max, (int64_t)1, 0.0f);
-                hipLaunchKernelGGL(k_mix_m0max, dim3(blocks_for(ns, 256)), dim3(256), 0, p->stream, ns,
+                hipLaunchKernelGGL(k_mix_m0max, dim3((unsigned)std::min<int64_t>(blocks_for(ns, 256), 256)),
+                                   dim3(256), 0, p->stream, ns,
                                    (const double2 *)p->d.mxy, W.m0max);
                 hipLaunchKernelGGL(k_mix_seg, dim3(blocks_for(ngroups * 16, 256)), dim3(256), 0,
                                    p->stream, ns, ngroups, (const double2 *)p->d.mxy,
