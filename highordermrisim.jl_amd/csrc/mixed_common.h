@@ -57,12 +57,6 @@ constexpr int kHOOut = 20;
 // offset can be added to the reduced FP64 phase in FP32 (error <= 1e-7 rev).
 // Segments with larger offsets use the all-FP64 phase body.
 constexpr float kHOBMax = 512.0f;
-// Instruction interleave of the per-group compute: after the tile loop the
-// scheduler is asked to alternate one MFMA with HOM_SCHED_VALU VALU
-// instructions (0 leaves the default schedule).
-#ifndef HOM_SCHED_VALU
-#define HOM_SCHED_VALU 0
-#endif
 #ifndef HOM_TILES
 #define HOM_TILES 2
 #endif
@@ -642,13 +636,6 @@ __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem
                 acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo, bh[c], acc[t][c], 0, 0, 0);
             }
         }
-#if HOM_SCHED_VALU > 0
-#pragma unroll
-        for (int i = 0; i < kTiles * (1 + NMH + 3 * NCB); ++i) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);               // one MFMA
-            __builtin_amdgcn_sched_group_barrier(0x002, HOM_SCHED_VALU, 0);  // then VALU
-        }
-#endif
     };
 
     const int64_t g0 = it.stile * o.gpt;
