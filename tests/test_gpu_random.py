@@ -23,7 +23,7 @@ from helpers import assert_isapprox, nrmse, print_case, require_gpu
 
 pytestmark = pytest.mark.gpu
 
-SEEDS = list(range(1000, 1016))
+SEEDS = list(range(1000, 1032))
 
 
 def random_case(seed):
