@@ -176,21 +176,26 @@ __global__ __launch_bounds__(256) void k_plain_excitation(
 // is the state's real or imaginary part, the B operand (re, im; -im, re) of
 // the sensitivity, so the 16 output columns of one MFMA tile are 8 coils x
 // (re, im).  One workgroup per (spin share, 32-sample x 32-coil channel
-// tile); the spins go in 16-spin chunks, and workgroup g of G takes chunks
-// g, g + G, g + 2G, ... so that the workgroups in flight read neighbouring
-// runs of every state and sensitivity row (owning contiguous spin tiles
-// instead ran 8-19 % slower).  G = ceil(Ns / kSpinTile): 512 workgroups for
-// 1M spins, the fastest measured (of 256-2048 for FP64, 512-4096 for FP32).
-// Wave w
-// owns sample tile w & 1 and coils 16 (w >> 1) .. +15 (two 16x16
-// accumulators); chunks pass through LDS spin-major, rows padded by one
+// tile); the spins go in chunks of kCoilChunk<T> spins (256 bytes of a row),
+// and workgroup g of G takes chunks g, g + G, g + 2G, ... so that the
+// workgroups in flight read neighbouring runs of every state and sensitivity
+// row (owning contiguous spin tiles instead ran 8-19 % slower).
+// G = ceil(Ns / kSpinTile): 512 workgroups for 1M spins, the fastest measured
+// (of 256-2048 for FP64, 512-4096 for FP32).  Wave w owns sample tile w & 1
+// and coils 16 (w >> 1) .. +15 (two 16x16 accumulators); chunks pass through
+// LDS spin-major, rows padded by one
 // element (conflict-free transposed stores and fragment reads),
 // double-buffered, with the global loads two chunks ahead of the compute.
 // The partial of workgroup g sums its chunks in a fixed order: the result is
 // deterministic.
 constexpr int64_t kSpinTile = 2048;
 constexpr int64_t kSpinTileMin = kSpinTile;
-constexpr int kCoilChunk = 16;
+// Spins per chunk: 256 bytes of one state or sensitivity row (16 FP64 or 32
+// FP32 complex values).  Measured on a 32-sample block of 1M spins and 32
+// coils: FP64 259 us with 16-spin and 278 us with 32-spin chunks, FP32 157 us
+// with 16 and 140 us with 32.
+template <typename T>
+constexpr int kCoilChunk = 256 / (int)sizeof(Cx<T>);
 
 template <typename T>
 struct CoilMfma;
@@ -217,7 +222,7 @@ __global__ __launch_bounds__(256) void k_coil_partials(
     int64_t ns, int64_t nadc, int64_t nc, const Cx<T> *__restrict__ states, int64_t lds,
     const Cx<T> *__restrict__ csm, int64_t ldc, Cx<T> *__restrict__ partials, int64_t nch) {
     using M = CoilMfma<T>;
-    constexpr int kC = kCoilChunk;
+    constexpr int kC = kCoilChunk<T>;
     __shared__ Cx<T> st[2][kC][33];  // [buffer][spin][sample]
     __shared__ Cx<T> cs[2][kC][33];  // [buffer][spin][coil]
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -232,8 +237,9 @@ __global__ __launch_bounds__(256) void k_coil_partials(
 
     // staging: element e = tid + 256 j of a chunk is spin e % kC, sample/coil e / kC;
     // two register sets (chunks i + 1 and i + 2 in flight while chunk i is in use)
+    constexpr int kJ = kC * 32 / 256;  // staged elements per thread and array
     struct Stage {
-        Cx<T> s[2], c[2];
+        Cx<T> s[kJ], c[kJ];
     };
     // Loads are unconditional, from clamped in-bounds addresses (ns, nadc and
     // nc are at least 1 here), and the bounds apply as selects at the store:
@@ -242,7 +248,7 @@ __global__ __launch_bounds__(256) void k_coil_partials(
     auto load = [&](Stage &r, int64_t i) {
         const int64_t sc = chunk0(i);
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < kJ; ++j) {
             const int e = tid + 256 * j, sl = e % kC, q = e / kC;
             const int64_t sp = std::min<int64_t>(sc + sl, s_hi - 1);
             r.s[j] = states[sp + std::min<int64_t>(k0 + q, nadc - 1) * lds];
@@ -252,7 +258,7 @@ __global__ __launch_bounds__(256) void k_coil_partials(
     auto store = [&](const Stage &r, int b, int64_t i) {
         const int64_t sc = chunk0(i);
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < kJ; ++j) {
             const int e = tid + 256 * j, sl = e % kC, q = e / kC;
             const bool in = i < nck && sc + sl < s_hi;
             st[b][sl][q] = (in && k0 + q < nadc) ? r.s[j] : Cx<T>{};
