@@ -52,7 +52,7 @@ int ensure_stats(hom_plan *p) {
 int ensure_partials(hom_plan *p, int64_t n_adc) {
     const int64_t need = coil_partial_elements(p->ns, n_adc, p->nc);
     if (need <= p->d.partial_capacity) return HOM_OK;
-    if (p->d.partials) hipFree(p->d.partials);
+    if (p->d.partials) (void)hipFree(p->d.partials);
     p->d.partials = nullptr;
     p->d.partial_capacity = 0;
     if (hipMalloc(&p->d.partials, (size_t)need * p->complex_size()) != hipSuccess) {
@@ -116,7 +116,7 @@ int block_common(hom_plan *p, int64_t first, int64_t last, int flags, void *out,
 }
 
 void free_ptr(void *&ptr) {
-    if (ptr) hipFree(ptr);
+    if (ptr) (void)hipFree(ptr);
     ptr = nullptr;
 }
 }  // namespace
@@ -136,6 +136,12 @@ int check_hip(hom_plan *p, hipError_t e, const char *what) {
     return set_error(p, HOM_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+int enter_device(hom_plan *p, bool sync) {
+    int rc = check_hip(p, hipSetDevice(p->device), "hipSetDevice");
+    if (!rc && sync && p->stream) rc = check_hip(p, hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+    return rc;
+}
+
 ScopedTimer::ScopedTimer(hom_plan *plan, int kernel_id) : p(plan), id(kernel_id) {
     if (!p || !p->timing) return;
     TimerSlot &s = p->timers[id];
@@ -145,13 +151,13 @@ ScopedTimer::ScopedTimer(hom_plan *plan, int kernel_id) : p(plan), id(kernel_id)
         s.starts.push_back(a);
         s.stops.push_back(b);
     }
-    hipEventRecord(s.starts[s.used], p->stream);
+    (void)hipEventRecord(s.starts[s.used], p->stream);  // timing only
     stop_ev = s.stops[s.used];
 }
 
 ScopedTimer::~ScopedTimer() {
     if (!stop_ev) return;
-    hipEventRecord(stop_ev, p->stream);
+    (void)hipEventRecord(stop_ev, p->stream);  // timing only
     p->timers[id].used++;
 }
 
@@ -250,21 +256,21 @@ static int hom_plan_create_impl(hom_plan **out, int device, int precision, int64
 
 int hom_plan_destroy(hom_plan *p) {
     if (!p) return HOM_OK;
-    hipSetDevice(p->device);
-    if (p->stream) hipStreamSynchronize(p->stream);
+    (void)hipSetDevice(p->device);
+    if (p->stream) (void)hipStreamSynchronize(p->stream);  // releases proceed regardless
     mixed_free(p);
     void **all[] = {&p->d.x,  &p->d.y,  &p->d.z,  &p->d.rho, &p->d.T1, &p->d.T2,  &p->d.dw,
                     &p->d.csm, &p->d.coef, &p->d.gx, &p->d.gy, &p->d.gz, &p->d.dt, &p->d.b1,
                     &p->d.df, &p->d.psi, &p->d.mxy, &p->d.mz, &p->d.states, &p->d.partials,
                     (void **)&p->d.stats, &p->signal};
     for (void **q : all) free_ptr(*q);
-    if (p->d.adc) hipFree(p->d.adc);
-    for (void *b : p->user_buffers) hipFree(b);
+    if (p->d.adc) (void)hipFree(p->d.adc);
+    for (void *b : p->user_buffers) (void)hipFree(b);
     for (auto &s : p->timers) {
-        for (auto e : s.starts) hipEventDestroy(e);
-        for (auto e : s.stops) hipEventDestroy(e);
+        for (auto e : s.starts) (void)hipEventDestroy(e);
+        for (auto e : s.stops) (void)hipEventDestroy(e);
     }
-    if (p->stream) hipStreamDestroy(p->stream);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
     return HOM_OK;
 }
@@ -274,8 +280,7 @@ static int hom_upload_phantom_impl(hom_plan *p, const double *x, const double *y
     if (!p) return HOM_ERR_ARGUMENT;
     if (p->ns > 0 && (!x || !y || !z || !rho || !T1 || !T2 || !dw))
         return set_error(p, HOM_ERR_ARGUMENT, "Every phantom field needs a host pointer.");
-    hipSetDevice(p->device);
-    hipStreamSynchronize(p->stream);
+    if (const int e = enter_device(p, true)) return e;
     int rc = HOM_OK;
     rc = rc ? rc : upload_real(p, &p->d.x, x, p->ns);
     rc = rc ? rc : upload_real(p, &p->d.y, y, p->ns);
@@ -308,8 +313,7 @@ static int hom_upload_csm_impl(hom_plan *p, const double *csm, int64_t ld) {
     if (ld < std::max<int64_t>(p->ns, 1))
         return set_error(p, HOM_ERR_DIMENSION, "The CSM leading dimension is smaller than the spin count.");
     if (p->ns > 0 && !csm) return set_error(p, HOM_ERR_ARGUMENT, "The CSM host pointer is NULL.");
-    hipSetDevice(p->device);
-    hipStreamSynchronize(p->stream);
+    if (const int e = enter_device(p, true)) return e;
     const size_t n = (size_t)(p->ns * p->nc);
     p->h_csm_colmax.assign((size_t)p->nc, 0.0);
     int rc = ensure_alloc(p, &p->d.csm, p->complex_size() * n);
@@ -365,8 +369,7 @@ static int hom_upload_sequence_impl(hom_plan *p, const hom_sequence_host *seq) {
         if (!seq->h[j]) return set_error(p, HOM_ERR_ARGUMENT, "All sixteen coefficient waveforms are required.");
     for (int64_t i = 0; i < ni; ++i)
         if (!(seq->dt[i] >= 0.0)) return set_error(p, HOM_ERR_ARGUMENT, "Time steps must be nonnegative.");
-    hipSetDevice(p->device);
-    hipStreamSynchronize(p->stream);
+    if (const int e = enter_device(p, true)) return e;
     p->h_t.assign(seq->t, seq->t + n);
     p->h_dt.assign(seq->dt ? seq->dt : seq->t, (seq->dt ? seq->dt : seq->t) + ni);
     p->h_gx.assign(seq->gx, seq->gx + n);
@@ -402,15 +405,14 @@ static int hom_upload_sequence_impl(hom_plan *p, const hom_sequence_host *seq) {
 int hom_reset_state(hom_plan *p) {
     if (!p) return HOM_ERR_ARGUMENT;
     if (!p->has_phantom) return set_error(p, HOM_ERR_STATE, "The phantom has not been uploaded.");
-    hipSetDevice(p->device);
+    if (const int e = enter_device(p, false)) return e;
     return check_hip(p, launch_reset_state(p), "state reset");
 }
 
 static int hom_set_state_impl(hom_plan *p, const double *mxy, const double *mz) {
     if (!p) return HOM_ERR_ARGUMENT;
     if (p->ns > 0 && (!mxy || !mz)) return set_error(p, HOM_ERR_ARGUMENT, "State pointers are NULL.");
-    hipSetDevice(p->device);
-    hipStreamSynchronize(p->stream);
+    if (const int e = enter_device(p, true)) return e;
     int rc = upload_real(p, &p->d.mxy, mxy, 2 * p->ns);
     return rc ? rc : upload_real(p, &p->d.mz, mz, p->ns);
 }
@@ -418,8 +420,7 @@ static int hom_set_state_impl(hom_plan *p, const double *mxy, const double *mz) 
 static int hom_get_state_impl(hom_plan *p, double *mxy, double *mz) {
     if (!p) return HOM_ERR_ARGUMENT;
     if (p->ns > 0 && (!mxy || !mz)) return set_error(p, HOM_ERR_ARGUMENT, "State pointers are NULL.");
-    hipSetDevice(p->device);
-    int rc = check_hip(p, hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+    int rc = enter_device(p, true);
     if (rc || p->ns == 0) return rc;
     const size_t ns = (size_t)p->ns;
     if (p->is_f32()) {
@@ -437,13 +438,13 @@ static int hom_get_state_impl(hom_plan *p, double *mxy, double *mz) {
 
 static int hom_precession_block_impl(hom_plan *p, int64_t first, int64_t last, int flags, void *out, int64_t ld) {
     if (!p) return HOM_ERR_ARGUMENT;
-    hipSetDevice(p->device);
+    if (const int e = enter_device(p, false)) return e;
     return block_common(p, first, last, flags, out, ld, false);
 }
 
 static int hom_excitation_block_impl(hom_plan *p, int64_t first, int64_t last, int flags, void *out, int64_t ld) {
     if (!p) return HOM_ERR_ARGUMENT;
-    hipSetDevice(p->device);
+    if (const int e = enter_device(p, false)) return e;
     return block_common(p, first, last, flags, out, ld, true);
 }
 
@@ -457,7 +458,7 @@ static int hom_reduce_signal_impl(hom_plan *p, const void *states, int64_t ld_st
     if (out_ld < n_adc) return set_error(p, HOM_ERR_DIMENSION, "The output leading dimension is too small.");
     if (p->ns > 0 && (!states || ld_states < p->ns))
         return set_error(p, HOM_ERR_DIMENSION, "States, CSM, and signal dimensions do not match.");
-    hipSetDevice(p->device);
+    if (const int e = enter_device(p, false)) return e;
     int rc = ensure_partials(p, n_adc);
     if (rc) return rc;
     return check_hip(p, launch_coil_reduce(p, states, ld_states, n_adc, out, out_ld), "coil reduction");
@@ -479,9 +480,9 @@ static int hom_run_impl(hom_plan *p, const int64_t *pf, const int64_t *pl, const
             return set_error(p, HOM_ERR_ARGUMENT, "Adjacent parts must share their boundary node.");
         total += nadc;
     }
-    hipSetDevice(p->device);
+    if ((rc = enter_device(p, false))) return rc;
     if (total > p->signal_capacity_rows) {
-        hipStreamSynchronize(p->stream);
+        if ((rc = check_hip(p, hipStreamSynchronize(p->stream), "hipStreamSynchronize"))) return rc;
         free_ptr(p->signal);
         p->signal_capacity_rows = 0;
         rc = ensure_alloc(p, &p->signal, p->complex_size() * (size_t)(total * p->nc));
@@ -503,8 +504,7 @@ static int hom_run_impl(hom_plan *p, const int64_t *pf, const int64_t *pl, const
 
 int hom_synchronize(hom_plan *p) {
     if (!p) return HOM_ERR_ARGUMENT;
-    hipSetDevice(p->device);
-    return check_hip(p, hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+    return enter_device(p, true);
 }
 
 int hom_signal_rows(hom_plan *p, int64_t *rows) {
@@ -524,8 +524,7 @@ int hom_download_signal(hom_plan *p, void *host, int64_t ld) {
     if (p->signal_rows == 0) return HOM_OK;
     if (!host) return set_error(p, HOM_ERR_ARGUMENT, "The host output pointer is NULL.");
     if (ld < p->signal_rows) return set_error(p, HOM_ERR_DIMENSION, "The host leading dimension is too small.");
-    hipSetDevice(p->device);
-    int rc = check_hip(p, hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+    int rc = enter_device(p, true);
     if (rc) return rc;
     const size_t es = p->complex_size();
     return check_hip(p, hipMemcpy2D(host, es * (size_t)ld, p->signal, es * (size_t)p->signal_rows,
@@ -535,8 +534,8 @@ int hom_download_signal(hom_plan *p, void *host, int64_t ld) {
 
 static int hom_buffer_alloc_impl(hom_plan *p, int64_t bytes, void **dev) {
     if (!p || !dev || bytes < 0) return HOM_ERR_ARGUMENT;
-    hipSetDevice(p->device);
     *dev = nullptr;
+    if (const int e = enter_device(p, false)) return e;
     if (bytes == 0) return HOM_OK;
     if (hipMalloc(dev, (size_t)bytes) != hipSuccess) {
         *dev = nullptr;
@@ -552,33 +551,29 @@ int hom_buffer_free(hom_plan *p, void *dev) {
     auto it = std::find(p->user_buffers.begin(), p->user_buffers.end(), dev);
     if (it == p->user_buffers.end())
         return set_error(p, HOM_ERR_ARGUMENT, "The buffer was not allocated by this plan.");
-    hipSetDevice(p->device);
-    hipStreamSynchronize(p->stream);
-    hipFree(dev);
+    const int rc = enter_device(p, true);  // the buffer is released either way
+    (void)hipFree(dev);
     p->user_buffers.erase(it);
-    return HOM_OK;
+    return rc;
 }
 
 int hom_buffer_upload(hom_plan *p, void *dev, const void *host, int64_t bytes) {
     if (!p || bytes < 0 || (bytes && (!dev || !host))) return HOM_ERR_ARGUMENT;
     if (!bytes) return HOM_OK;
-    hipSetDevice(p->device);
-    hipStreamSynchronize(p->stream);
+    if (const int e = enter_device(p, true)) return e;
     return check_hip(p, hipMemcpy(dev, host, (size_t)bytes, hipMemcpyHostToDevice), "hipMemcpy");
 }
 
 int hom_buffer_download(hom_plan *p, void *host, const void *dev, int64_t bytes) {
     if (!p || bytes < 0 || (bytes && (!dev || !host))) return HOM_ERR_ARGUMENT;
     if (!bytes) return HOM_OK;
-    hipSetDevice(p->device);
-    int rc = check_hip(p, hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+    int rc = enter_device(p, true);
     return rc ? rc : check_hip(p, hipMemcpy(host, dev, (size_t)bytes, hipMemcpyDeviceToHost), "hipMemcpy");
 }
 
 static int hom_plan_set_timing_impl(hom_plan *p, int enable) {
     if (!p) return HOM_ERR_ARGUMENT;
-    hipSetDevice(p->device);
-    hipStreamSynchronize(p->stream);
+    if (const int e = enter_device(p, true)) return e;
     p->timing = enable != 0;
     for (auto &s : p->timers) {
         s.used = 0;
@@ -597,8 +592,7 @@ int hom_plan_mixed_stats(const hom_plan *p, int64_t *segments, int64_t *fp64_seg
 
 static int hom_plan_kernel_time_impl(hom_plan *p, int id, double *total_ms, int64_t *launches) {
     if (!p || id < 0 || id >= HOM_K_COUNT || !total_ms || !launches) return HOM_ERR_ARGUMENT;
-    hipSetDevice(p->device);
-    int rc = check_hip(p, hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+    int rc = enter_device(p, true);
     if (rc) return rc;
     TimerSlot &s = p->timers[id];
     for (size_t i = 0; i < s.used; ++i) {

@@ -132,6 +132,9 @@ int guarded_host(F &&f) {
     }
 }
 int check_hip(hom_plan *p, hipError_t e, const char *what);
+// Makes the plan's device current and, with sync, waits for its stream (so
+// that an earlier asynchronous failure is reported here); returns a status.
+int enter_device(hom_plan *p, bool sync);
 
 // ---- plain kernels (plain_kernels.hip) ----
 // Block [first, last] of free precession / excitation for all spins, writing

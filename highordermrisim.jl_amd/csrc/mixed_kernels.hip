@@ -278,7 +278,7 @@ hipError_t launch_main(int ncb, bool fallback, int nmh, int NI, hipStream_t st, 
 
 template <typename T>
 int dev_upload(hom_plan *p, T **dst, const std::vector<T> &src) {
-    if (*dst) hipFree(*dst);
+    if (*dst) (void)hipFree(*dst);
     *dst = nullptr;
     if (src.empty()) return HOM_OK;
     if (hipMalloc((void **)dst, sizeof(T) * src.size()) != hipSuccess) {
@@ -344,7 +344,7 @@ void host_split16(double v, _Float16 &hi, _Float16 &lo) {
 
 template <typename T>
 void free_dev(T *&q) {
-    if (q) hipFree(q);
+    if (q) (void)hipFree(q);
     q = nullptr;
 }
 
@@ -548,7 +548,7 @@ void build_tables(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, i
 void mixed_free(hom_plan *p) {
     if (!p || !p->mixed) return;
     MixedWorkspace *W = p->mixed;
-    hipStreamSynchronize(p->stream);
+    (void)hipStreamSynchronize(p->stream);  // releases proceed regardless
     free_dev(W->basisA); free_dev(W->basis64); free_dev(W->aho); free_dev(W->cdec);
     free_dev(W->bop); free_dev(W->csc); free_dev(W->th0); free_dev(W->lm); free_dev(W->m0max);
     free_dev(W->seed); free_dev(W->kr); free_dev(W->k64); free_dev(W->bho); free_dev(W->anc);
@@ -572,7 +572,7 @@ int mixed_run(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *
 
     // Spin-side operands that depend only on the phantom, the CSM and the mask.
     if (W.basis_flags != mflags || W.ngroups != ngroups || W.NCB != NCB) {
-        hipStreamSynchronize(p->stream);
+        if (const int e = check_hip(p, hipStreamSynchronize(p->stream), "hipStreamSynchronize")) return e;
         W.terms = make_terms(mflags);
         W.split = make_split(mflags, p->h_basis_max);
         W.NI = (W.terms.n + 3) / 4;
@@ -625,7 +625,7 @@ int mixed_run(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *
         key.push_back(rf[b]);
     }
     if (W.table_flags != mflags || W.table_parts != key) {
-        hipStreamSynchronize(p->stream);
+        if (const int e = check_hip(p, hipStreamSynchronize(p->stream), "hipStreamSynchronize")) return e;
         SeqHost sq;
         sq.nn = p->nn;
         sq.gamma = p->gamma;
@@ -658,7 +658,7 @@ int mixed_run(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *
             need_seed = std::max<int64_t>(need_seed, s.nab * ngroups * 16);
         }
     if (need > W.partial_cap || need_seed > W.seed_cap) {
-        hipStreamSynchronize(p->stream);
+        if (const int e = check_hip(p, hipStreamSynchronize(p->stream), "hipStreamSynchronize")) return e;
         if (need > W.partial_cap) {
             free_dev(W.partials);
             W.partial_cap = 0;
@@ -691,8 +691,8 @@ int mixed_run(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *
             if (s.nadc) {
                 const int64_t need_p = coil_partial_elements(ns, s.nadc, nc);
                 if (need_p > p->d.partial_capacity) {
-                    hipStreamSynchronize(p->stream);
-                    if (p->d.partials) hipFree(p->d.partials);
+                    if (const int e = check_hip(p, hipStreamSynchronize(p->stream), "hipStreamSynchronize")) return e;
+                    if (p->d.partials) (void)hipFree(p->d.partials);
                     p->d.partials = nullptr;
                     p->d.partial_capacity = 0;
                     if (hipMalloc(&p->d.partials, (size_t)need_p * p->complex_size()) != hipSuccess)

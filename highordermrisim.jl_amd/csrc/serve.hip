@@ -39,8 +39,9 @@ namespace hom {
 namespace serve {
 
 constexpr int kMaxJobs = HOM_SERVE_MAX_ACTIVE;
-constexpr int kSplitClasses = 12;  // nmh 0..2 x ncb 1..4
-constexpr int kClasses = 28;       // + all-FP64: NI 1..4 x ncb 1..4 (run_class cases)
+// run_class cases 0..11: split phase, nmh 0..2 x ncb 1..4; 12..27: all-FP64,
+// NI 1..4 x ncb 1..4
+constexpr int kSplitClasses = 12;
 constexpr int kMaxMain = 4 * kMaxJobs;
 constexpr int kLightGrid = 1024;
 constexpr int64_t kAlign = 256;
@@ -884,7 +885,7 @@ int launch_step(hom_server *s) {
         e = hipStreamEndCapture(s->stream, &g);
         if (e != hipSuccess) return hip_fail(s, e, "hipStreamEndCapture");
         e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
+        (void)hipGraphDestroy(g);
         if (e != hipSuccess) return hip_fail(s, e, "hipGraphInstantiate");
         it = s->graphs.emplace(key, ex).first;
         s->graphs_captured++;
@@ -908,9 +909,9 @@ void release_resident_locked(hom_server *s, Job &j) {
                 }
         }
     };
-    drop(s->phantoms, j.ph, [](ResPhantom *p) { hipFree(p->d); });
-    drop(s->csms, j.cs, [](ResCsm *c) { hipFree(c->d); });
-    drop(s->seqs, j.sq, [](ResSeq *q) { hipFree(q->d); hipFree(q->dadc); });
+    drop(s->phantoms, j.ph, [](ResPhantom *p) { (void)hipFree(p->d); });
+    drop(s->csms, j.cs, [](ResCsm *c) { (void)hipFree(c->d); });
+    drop(s->seqs, j.sq, [](ResSeq *q) { (void)hipFree(q->d); (void)hipFree(q->dadc); });
     j.ph = nullptr;
     j.cs = nullptr;
     j.sq = nullptr;
@@ -944,7 +945,7 @@ void fail_active_locked(hom_server *s, const std::string &msg) {
 // One scheduler step.  Runs under step_mu (never under mu while on the GPU).
 int step_once(hom_server *s, int *worked) {
     *worked = 0;
-    hipSetDevice(s->cfg.device);
+    if (const hipError_t e = hipSetDevice(s->cfg.device); e != hipSuccess) return hip_fail(s, e, "hipSetDevice");
     std::vector<Job *> admitted;
     std::vector<Job *> act;
     {
@@ -1110,7 +1111,7 @@ int guarded_server(hom_server *s, F &&f) {
 }
 
 void worker_main(hom_server *s) {
-    hipSetDevice(s->cfg.device);
+    (void)hipSetDevice(s->cfg.device);  // each step sets and checks it again
     uint64_t blocked_at = ~(uint64_t)0;  // epoch at which the queue head could not be admitted
     for (;;) {
         {
@@ -1205,11 +1206,11 @@ static int hom_server_create_impl(hom_server **out, const hom_server_config *con
     s->cfg = c;
     auto dev_fail = [&](const char *m) {
         g_server_create_error = m;
-        if (s->stream) hipStreamDestroy(s->stream);
-        if (s->dev_base) hipFree(s->dev_base);
-        if (s->host_base) hipHostFree(s->host_base);
-        if (s->h_desc) hipHostFree(s->h_desc);
-        if (s->d_desc) hipFree(s->d_desc);
+        if (s->stream) (void)hipStreamDestroy(s->stream);
+        if (s->dev_base) (void)hipFree(s->dev_base);
+        if (s->host_base) (void)hipHostFree(s->host_base);
+        if (s->h_desc) (void)hipHostFree(s->h_desc);
+        if (s->d_desc) (void)hipFree(s->d_desc);
         return HOM_ERR_MEMORY;
     };
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
@@ -1239,20 +1240,21 @@ int hom_server_destroy(hom_server *s) {
     }
     s->cv_work.notify_all();
     if (s->worker.joinable()) s->worker.join();
-    hipSetDevice(s->cfg.device);
-    hipStreamSynchronize(s->stream);
-    for (auto &kv : s->graphs) hipGraphExecDestroy(kv.second);
-    for (auto &kv : s->phantoms) hipFree(kv.second->d);
-    for (auto &kv : s->csms) hipFree(kv.second->d);
+    // releases proceed regardless of earlier device errors
+    (void)hipSetDevice(s->cfg.device);
+    (void)hipStreamSynchronize(s->stream);
+    for (auto &kv : s->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto &kv : s->phantoms) (void)hipFree(kv.second->d);
+    for (auto &kv : s->csms) (void)hipFree(kv.second->d);
     for (auto &kv : s->seqs) {
-        hipFree(kv.second->d);
-        hipFree(kv.second->dadc);
+        (void)hipFree(kv.second->d);
+        (void)hipFree(kv.second->dadc);
     }
-    hipStreamDestroy(s->stream);
-    hipFree(s->dev_base);
-    hipHostFree(s->host_base);
-    hipHostFree(s->h_desc);
-    hipFree(s->d_desc);
+    (void)hipStreamDestroy(s->stream);
+    (void)hipFree(s->dev_base);
+    (void)hipHostFree(s->host_base);
+    (void)hipHostFree(s->h_desc);
+    (void)hipFree(s->d_desc);
     delete s;
     return HOM_OK;
 }
@@ -1276,7 +1278,7 @@ static int hom_server_put_phantom_impl(hom_server *s, int64_t ns, const double *
     for (int64_t i = 0; i < ns; ++i)
         if (!(T1[i] > 0.0) || !(T2[i] > 0.0))
             return fail(s, HOM_ERR_ARGUMENT, "T1 and T2 must be positive.");
-    hipSetDevice(s->cfg.device);
+    if (const hipError_t e = hipSetDevice(s->cfg.device); e != hipSuccess) return hip_fail(s, e, "hipSetDevice");
     auto ph = std::make_unique<ResPhantom>();
     ph->ns = ns;
     if (hipMalloc((void **)&ph->d, sizeof(double) * 7 * (size_t)ns) != hipSuccess)
@@ -1284,7 +1286,7 @@ static int hom_server_put_phantom_impl(hom_server *s, int64_t ns, const double *
     for (int i = 0; i < 7; ++i) {
         hipError_t e = hipMemcpy(ph->d + (size_t)i * ns, src[i], sizeof(double) * ns, hipMemcpyHostToDevice);
         if (e != hipSuccess) {
-            hipFree(ph->d);
+            (void)hipFree(ph->d);
             return hip_fail(s, e, "phantom upload");
         }
     }
@@ -1314,7 +1316,7 @@ static int hom_server_put_csm_impl(hom_server *s, int64_t phantom, int64_t nc, c
     }
     if (nc < 1) return fail(s, HOM_ERR_DIMENSION, "A CSM needs at least one coil.");
     if (ld < ns) return fail(s, HOM_ERR_DIMENSION, "The CSM leading dimension is smaller than the spin count.");
-    hipSetDevice(s->cfg.device);
+    if (const hipError_t e = hipSetDevice(s->cfg.device); e != hipSuccess) return hip_fail(s, e, "hipSetDevice");
     auto cs = std::make_unique<ResCsm>();
     cs->phantom = phantom;
     cs->nc = nc;
@@ -1326,7 +1328,7 @@ static int hom_server_put_csm_impl(hom_server *s, int64_t phantom, int64_t nc, c
         return fail(s, HOM_ERR_MEMORY, "Device allocation failed for the CSM.");
     hipError_t e = hipMemcpy(cs->d, cs->host.data(), sizeof(double2) * (size_t)(ns * nc), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        hipFree(cs->d);
+        (void)hipFree(cs->d);
         return hip_fail(s, e, "CSM upload");
     }
     const ResCsm &csr = *cs;
@@ -1350,7 +1352,7 @@ static int hom_server_put_sequence_impl(hom_server *s, const hom_sequence_host *
     for (int64_t i = 0; i < ni; ++i)
         if (!(seq->dt[i] >= 0.0)) return fail(s, HOM_ERR_ARGUMENT, "Time steps must be nonnegative.");
     if (!(gamma > 0.0)) return fail(s, HOM_ERR_ARGUMENT, "gamma must be positive.");
-    hipSetDevice(s->cfg.device);
+    if (const hipError_t e = hipSetDevice(s->cfg.device); e != hipSuccess) return hip_fail(s, e, "hipSetDevice");
     auto q = std::make_unique<ResSeq>();
     q->nn = n;
     q->gamma = gamma;
@@ -1376,14 +1378,14 @@ static int hom_server_put_sequence_impl(hom_server *s, const hom_sequence_host *
     std::copy(seq->dt, seq->dt + ni, h.begin() + 7 * n);
     if (hipMalloc((void **)&q->d, sizeof(double) * h.size()) != hipSuccess ||
         hipMalloc((void **)&q->dadc, (size_t)n) != hipSuccess) {
-        if (q->d) hipFree(q->d);
+        if (q->d) (void)hipFree(q->d);
         return fail(s, HOM_ERR_MEMORY, "Device allocation failed for the sequence.");
     }
     hipError_t e = hipMemcpy(q->d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(q->dadc, q->adc.data(), (size_t)n, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        hipFree(q->d);
-        hipFree(q->dadc);
+        (void)hipFree(q->d);
+        (void)hipFree(q->dadc);
         return hip_fail(s, e, "sequence upload");
     }
     std::lock_guard<std::mutex> lk(s->mu);
@@ -1394,7 +1396,7 @@ static int hom_server_put_sequence_impl(hom_server *s, const hom_sequence_host *
 
 static int hom_server_release_impl(hom_server *s, int kind, int64_t id) {
     if (!s) return HOM_ERR_ARGUMENT;
-    hipSetDevice(s->cfg.device);
+    if (const hipError_t e = hipSetDevice(s->cfg.device); e != hipSuccess) return hip_fail(s, e, "hipSetDevice");
     std::lock_guard<std::mutex> lk(s->mu);
     auto go = [&](auto &map, auto freer) -> int {
         auto it = map.find(id);
@@ -1411,12 +1413,12 @@ static int hom_server_release_impl(hom_server *s, int kind, int64_t id) {
             for (auto &kv : s->csms)
                 if (kv.second->phantom == id && !kv.second->released)
                     return fail_locked(s, HOM_ERR_STATE, "Release the phantom's CSMs first.");
-            return go(s->phantoms, [](ResPhantom *p) { hipFree(p->d); });
-        case HOM_RES_CSM: return go(s->csms, [](ResCsm *c) { hipFree(c->d); });
+            return go(s->phantoms, [](ResPhantom *p) { (void)hipFree(p->d); });
+        case HOM_RES_CSM: return go(s->csms, [](ResCsm *c) { (void)hipFree(c->d); });
         case HOM_RES_SEQUENCE:
             return go(s->seqs, [](ResSeq *q) {
-                hipFree(q->d);
-                hipFree(q->dadc);
+                (void)hipFree(q->d);
+                (void)hipFree(q->dadc);
             });
         default: return fail_locked(s, HOM_ERR_ARGUMENT, "Unknown resident kind.");
     }
@@ -1641,7 +1643,7 @@ static int hom_server_autotune_impl(hom_server *s, const hom_request *probe, con
         if (g <= 0 || g % 8) return fail(s, HOM_ERR_ARGUMENT, "Grid sizes must be positive multiples of 8.");
     if (gpts.empty() || grids.empty()) return fail(s, HOM_ERR_ARGUMENT, "The candidate lists are empty.");
     std::lock_guard<std::mutex> step(s->step_mu);
-    hipSetDevice(s->cfg.device);
+    if (const hipError_t e = hipSetDevice(s->cfg.device); e != hipSuccess) return hip_fail(s, e, "hipSetDevice");
     Job j;
     {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -1703,17 +1705,20 @@ static int hom_server_autotune_impl(hom_server *s, const hom_request *probe, con
                 if (e == hipSuccess) {
                     // prep kernels of the chunk (untimed), then the fused kernel (timed)
                     enqueue_prologue(s);
-                    hipEventRecord(e0, s->stream);
-                    enqueue_fused(s, grid);
-                    hipEventRecord(e1, s->stream);
-                    e = hipStreamSynchronize(s->stream);
+                    e = hipEventRecord(e0, s->stream);
+                    if (e == hipSuccess) {
+                        enqueue_fused(s, grid);
+                        e = hipEventRecord(e1, s->stream);
+                    }
+                    const hipError_t es = hipStreamSynchronize(s->stream);
+                    if (e == hipSuccess) e = es;
                 }
+                float ms = 0.0f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
                 if (e != hipSuccess) {
                     rc = hip_fail(s, e, "autotune run");
                     break;
                 }
-                float ms = 0.0f;
-                hipEventElapsedTime(&ms, e0, e1);
                 if (t_min < 0 || ms < t_min) t_min = ms;
             }
             if (!rc && (best < 0 || t_min < best)) {
@@ -1723,16 +1728,18 @@ static int hom_server_autotune_impl(hom_server *s, const hom_request *probe, con
             }
         }
     }
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
-    hipStreamSynchronize(s->stream);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    const hipError_t es = hipStreamSynchronize(s->stream);
     std::lock_guard<std::mutex> lk(s->mu);
     s->dev_arena.release(j.dev_off, j.dev_bytes);
     release_resident_locked(s, j);
     if (rc) return rc;
+    if (es != hipSuccess)
+        return fail_locked(s, HOM_ERR_DEVICE, std::string("hipStreamSynchronize: ") + hipGetErrorString(es));
     s->cfg.gpt = bg;
     s->cfg.main_grid = bgrid;
-    for (auto &kv : s->graphs) hipGraphExecDestroy(kv.second);
+    for (auto &kv : s->graphs) (void)hipGraphExecDestroy(kv.second);
     s->graphs.clear();
     if (best_gpt) *best_gpt = bg;
     if (best_grid) *best_grid = bgrid;
