@@ -128,6 +128,7 @@ def main():
     ns, nc = len(obj), obj.n_coils
     spin_samples = ns * total_adc
     max_adc = max(seq.block_adc(a, b) for a, b in zip(parts[0], parts[1]))
+    max_rf_adc = max((seq.block_adc(a, b) for a, b, r in zip(*parts) if r), default=0)
     log(f"workload {cfg.name}: spins={ns} coils={nc} nodes={seq.n_nodes} adc={total_adc} "
         f"parts={len(parts[0])} built in {time.perf_counter() - t0:.1f}s")
 
@@ -222,36 +223,29 @@ def main():
     # ---------------- plain GPU-FP64 drop-in on the same workload -----------------
     plain = None
     if not args.no_plain:
-        with Plan("f64", ns, nc, seq.n_nodes, max_adc) as pp:
-            pp.upload_phantom(obj)
-            pp.upload_sequence(seq)
-            log("plain f64 drop-in: warmup on the first interleave")
-            first_il = [arr[:2] for arr in parts]
-            pp.reset_state()
-            pp.run(first_il, method)
-            pp.synchronize()
-            pp.reset_state()
-            pp.synchronize()
-            a = time.perf_counter()
-            pp.run(parts, method)
-            pp.synchronize()
-            dt = time.perf_counter() - a
+        def time_plain(precision, mode):
+            """One full run of the per-block drop-in after a one-interleave warmup."""
+            with Plan(precision, ns, nc, seq.n_nodes, max_adc, max_rf_adc=max_rf_adc,
+                      coil_combination=mode) as pp:
+                pp.upload_phantom(obj)
+                pp.upload_sequence(seq)
+                pp.reset_state()
+                pp.run([arr[:2] for arr in parts], method)
+                pp.reset_state()
+                pp.synchronize()
+                a = time.perf_counter()
+                pp.run(parts, method)
+                pp.synchronize()
+                dt = time.perf_counter() - a
+            log(f"plain {precision} {mode}: {spin_samples / dt:.4e} spin-samples/s ({dt:.2f}s)")
+            return dt
+
+        dt = time_plain("f64", "gemm")
         plain = {"value": spin_samples / dt, "unit": "spin-samples/s", "block_length": L,
-                 "seconds": dt}
-        log(f"plain f64: {plain['value']:.4e} spin-samples/s ({dt:.2f}s)")
-        with Plan("f32", ns, nc, seq.n_nodes, max_adc) as pp:
-            pp.upload_phantom(obj)
-            pp.upload_sequence(seq)
-            pp.reset_state()
-            pp.run([arr[:2] for arr in parts], method)
-            pp.reset_state()
-            pp.synchronize()
-            a = time.perf_counter()
-            pp.run(parts, method)
-            pp.synchronize()
-            dt32 = time.perf_counter() - a
-        plain["f32_value"] = spin_samples / dt32
-        log(f"plain f32: {plain['f32_value']:.4e} spin-samples/s ({dt32:.2f}s)")
+                 "coil_combination": "gemm", "seconds": dt,
+                 "f32_value": spin_samples / time_plain("f32", "gemm"),
+                 "f64_fused_value": spin_samples / time_plain("f64", "fused_tiled"),
+                 "f32_fused_value": spin_samples / time_plain("f32", "fused_tiled")}
 
     # ---------------- accuracy + CPU baseline on a spin subsample -----------------
     accuracy, cpu = None, None
@@ -330,6 +324,8 @@ def main():
         "cpu_baseline": cpu,
         "plain_f64_dropin": plain,
         "speedup_vs_plain_f64": (value / plain["value"]) if plain else None,
+        "speedup_vs_fastest_plain_f64": (value / max(plain["value"], plain["f64_fused_value"]))
+        if plain else None,
         "accuracy": accuracy,
         "latency": latency,
         "serving": serving,

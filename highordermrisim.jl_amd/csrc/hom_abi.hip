@@ -49,8 +49,7 @@ int ensure_stats(hom_plan *p) {
                         sizeof(unsigned long long) * (size_t)std::max<int64_t>(17, p->nc));
 }
 
-int ensure_partials(hom_plan *p, int64_t n_adc) {
-    const int64_t need = coil_partial_elements(p->ns, n_adc, p->nc);
+int ensure_partial_elements(hom_plan *p, int64_t need) {
     if (need <= p->d.partial_capacity) return HOM_OK;
     if (p->d.partials) (void)hipFree(p->d.partials);
     p->d.partials = nullptr;
@@ -61,6 +60,10 @@ int ensure_partials(hom_plan *p, int64_t n_adc) {
     }
     p->d.partial_capacity = need;
     return HOM_OK;
+}
+
+int ensure_partials(hom_plan *p, int64_t n_adc) {
+    return ensure_partial_elements(p, coil_partial_elements(p->ns, n_adc, p->nc));
 }
 
 int require_inputs(hom_plan *p) {
@@ -87,6 +90,9 @@ int check_block(hom_plan *p, int64_t first, int64_t last, bool rf, int64_t *nadc
     *nadc = host_block_adc(p, first, last);
     if (*nadc > p->max_adc)
         return set_error(p, HOM_ERR_DIMENSION, "The block acquires more samples than max_adc.");
+    // a fused plan's state buffer holds max_rf_adc samples (ADC during RF only)
+    if (rf && p->fused() && *nadc > p->max_rf_adc)
+        return set_error(p, HOM_ERR_DIMENSION, "The excitation block acquires more samples than max_rf_adc.");
     return HOM_OK;
 }
 
@@ -102,6 +108,12 @@ int block_common(hom_plan *p, int64_t first, int64_t last, int flags, void *out,
         return set_error(p, HOM_ERR_ARGUMENT, "A block with ADC samples needs an output buffer.");
     if (nadc > 0 && ld < nadc)
         return set_error(p, HOM_ERR_DIMENSION, "The output leading dimension is smaller than the block's ADC count.");
+    if (!rf && p->fused()) {
+        if (nadc > 0 && (rc = ensure_partial_elements(p, fused_partial_elements(p->ns, nadc, p->nc))))
+            return rc;
+        return check_hip(p, launch_fused_precession(p, first, last, flags, nadc, out, ld),
+                         "fused precession kernel");
+    }
     if (nadc > 0) {
         rc = ensure_partials(p, nadc);
         if (rc) return rc;
@@ -244,7 +256,9 @@ static int hom_plan_create_impl(hom_plan **out, int device, int precision, int64
     int rc = HOM_OK;
     rc |= ensure_alloc(p, &p->d.mxy, Cb * ns);
     rc |= ensure_alloc(p, &p->d.mz, R * ns);
-    rc |= ensure_alloc(p, &p->d.states, Cb * ns * (size_t)max_adc);
+    // ADC states: every block's in the per-block path, only those acquired
+    // during RF when the fused kernel takes the precession blocks
+    rc |= ensure_alloc(p, &p->d.states, Cb * ns * (size_t)(p->fused() ? max_rf_adc : max_adc));
     if (rc) {
         std::string msg = p->err;
         hom_plan_destroy(p);

@@ -101,6 +101,10 @@ struct hom_plan {
     std::vector<void *> user_buffers;
 
     bool is_f32() const { return precision == HOM_F32; }
+    // f32/f64 plan whose precession blocks run the fused kernel
+    bool fused() const {
+        return precision != HOM_MIXED && (coil_mode == HOM_COIL_FUSED_WARP || coil_mode == HOM_COIL_FUSED_TILED);
+    }
     size_t real_size() const { return is_f32() ? 4 : 8; }
     size_t complex_size() const { return 2 * real_size(); }
 };
@@ -145,6 +149,9 @@ hipError_t launch_plain_excitation(hom_plan *p, int64_t first, int64_t last, int
 hipError_t launch_coil_reduce(hom_plan *p, const void *states, int64_t ld_states, int64_t n_adc,
                               void *out, int64_t out_ld);
 int64_t coil_partial_elements(int64_t ns, int64_t n_adc, int64_t nc);
+int64_t fused_partial_elements(int64_t ns, int64_t n_adc, int64_t nc);
+hipError_t launch_fused_precession(hom_plan *p, int64_t first, int64_t last, int flags,
+                                   int64_t n_adc, void *out, int64_t out_ld);
 hipError_t launch_fill_complex(hom_plan *p, void *out, int64_t rows, int64_t cols, int64_t ld);
 hipError_t launch_reset_state(hom_plan *p);
 // Upload statistics as bit patterns of nonnegative doubles (out zeroed by the

@@ -25,7 +25,7 @@ HOM_OK, HOM_ERR_ARGUMENT, HOM_ERR_DIMENSION, HOM_ERR_DEVICE, HOM_ERR_MEMORY, HOM
 PRECISIONS = {"f32": 0, "f64": 1, "mixed": 2}
 COIL_MODES = {"gemm": 0, "reduction": 1, "fused_warp": 2, "fused_tiled": 3}
 KERNELS = {"plain_precession": 0, "plain_excitation": 1, "coil_partials": 2, "coil_finalize": 3,
-           "mixed_segment": 4, "mixed_finalize": 5, "mixed_prep": 6}
+           "mixed_segment": 4, "mixed_finalize": 5, "mixed_prep": 6, "fused_precession": 7}
 
 # Every symbol declared in include/hom_abi.h.
 ABI_SYMBOLS = (

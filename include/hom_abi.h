@@ -73,7 +73,8 @@ enum hom_kernel_id {
     HOM_K_MIXED_SEGMENT = 4,
     HOM_K_MIXED_FINALIZE = 5,
     HOM_K_MIXED_PREP = 6,
-    HOM_K_COUNT = 7
+    HOM_K_FUSED_PRECESSION = 7,
+    HOM_K_COUNT = 8
 };
 
 typedef struct hom_plan hom_plan;
@@ -99,7 +100,11 @@ const char *hom_last_error(const hom_plan *plan);
 /* Replaces the HighOrderGPUWorkspace constructor (reference
  * src/simulation/kernels/HighOrderGPU.jl:16-58): same argument checks
  * (0 <= max_rf_adc <= max_adc, known coil mode) plus the device, precision,
- * problem sizes and gamma (Hz/T), which the Julia side carries in its types. */
+ * problem sizes and gamma (Hz/T), which the Julia side carries in its types.
+ * As in the reference, an f32/f64 plan in a fused coil mode sizes its
+ * ADC-state buffer for max_rf_adc samples: its precession blocks run the
+ * fused kernel (FusedCoilGPU.jl:70-133) and only excitation blocks store
+ * states, each at most max_rf_adc of them. */
 int hom_plan_create(hom_plan **plan, int device, int precision, int64_t n_spins, int64_t n_coils,
                     int64_t n_nodes, int64_t max_adc, int64_t max_rf_adc, double gamma,
                     int coil_combination);
