@@ -23,7 +23,7 @@ int upload_vec(hom_plan *p, void **dst, const double *src, int64_t n) {
     std::vector<T> tmp((size_t)n);
     for (int64_t i = 0; i < n; ++i) tmp[(size_t)i] = (T)src[i];
     if (!*dst) {
-        hipError_t e = hipMalloc(dst, sizeof(T) * (size_t)n);
+        hipError_t e = hom::dev_malloc(dst, sizeof(T) * (size_t)n);
         if (e != hipSuccess) return set_error(p, HOM_ERR_MEMORY, "hipMalloc failed for an input array");
     }
     return check_hip(p, hipMemcpy(*dst, tmp.data(), sizeof(T) * (size_t)n, hipMemcpyHostToDevice),
@@ -36,7 +36,7 @@ int upload_real(hom_plan *p, void **dst, const double *src, int64_t n) {
 
 int ensure_alloc(hom_plan *p, void **ptr, size_t bytes) {
     if (*ptr || bytes == 0) return HOM_OK;
-    if (hipMalloc(ptr, bytes) != hipSuccess) {
+    if (hom::dev_malloc(ptr, bytes) != hipSuccess) {
         *ptr = nullptr;
         return set_error(p, HOM_ERR_MEMORY, "device allocation failed");
     }
@@ -54,7 +54,7 @@ int ensure_partial_elements(hom_plan *p, int64_t need) {
     if (p->d.partials) (void)hipFree(p->d.partials);
     p->d.partials = nullptr;
     p->d.partial_capacity = 0;
-    if (hipMalloc(&p->d.partials, (size_t)need * p->complex_size()) != hipSuccess) {
+    if (hom::dev_malloc(&p->d.partials, (size_t)need * p->complex_size()) != hipSuccess) {
         p->d.partials = nullptr;
         return set_error(p, HOM_ERR_MEMORY, "device allocation failed for coil partials");
     }
@@ -405,7 +405,7 @@ static int hom_upload_sequence_impl(hom_plan *p, const hom_sequence_host *seq) {
     rc = rc ? rc : upload_real(p, &p->d.coef, p->h_coef.data(), 16 * n);
     if (ni > 0) rc = rc ? rc : upload_real(p, &p->d.dt, seq->dt, ni);
     if (rc) return rc;
-    if (!p->d.adc && hipMalloc((void **)&p->d.adc, (size_t)n) != hipSuccess) {
+    if (!p->d.adc && hom::dev_malloc((void **)&p->d.adc, (size_t)n) != hipSuccess) {
         p->d.adc = nullptr;
         return set_error(p, HOM_ERR_MEMORY, "device allocation failed");
     }
@@ -551,7 +551,7 @@ static int hom_buffer_alloc_impl(hom_plan *p, int64_t bytes, void **dev) {
     *dev = nullptr;
     if (const int e = enter_device(p, false)) return e;
     if (bytes == 0) return HOM_OK;
-    if (hipMalloc(dev, (size_t)bytes) != hipSuccess) {
+    if (hom::dev_malloc(dev, (size_t)bytes) != hipSuccess) {
         *dev = nullptr;
         return set_error(p, HOM_ERR_MEMORY, "device allocation failed");
     }

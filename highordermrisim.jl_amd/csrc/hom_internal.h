@@ -14,6 +14,19 @@
 
 namespace hom {
 
+// hipMalloc that leaves nothing behind when it fails: the error is taken out
+// of the thread's last-error slot, so a later hipGetLastError() after a
+// kernel launch does not report this allocation failure as a launch failure.
+template <typename P>
+inline hipError_t dev_malloc(P **ptr, size_t bytes) {
+    const hipError_t e = hipMalloc(reinterpret_cast<void **>(ptr), bytes);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        (void)hipGetLastError();
+    }
+    return e;
+}
+
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 constexpr int kTermDW = 16;  // term code of the off-resonance term
 

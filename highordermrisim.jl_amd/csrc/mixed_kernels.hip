@@ -281,7 +281,7 @@ int dev_upload(hom_plan *p, T **dst, const std::vector<T> &src) {
     if (*dst) (void)hipFree(*dst);
     *dst = nullptr;
     if (src.empty()) return HOM_OK;
-    if (hipMalloc((void **)dst, sizeof(T) * src.size()) != hipSuccess) {
+    if (hom::dev_malloc((void **)dst, sizeof(T) * src.size()) != hipSuccess) {
         *dst = nullptr;
         return set_error(p, HOM_ERR_MEMORY, "device allocation failed for mixed-path tables");
     }
@@ -581,15 +581,15 @@ int mixed_run(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *
         free_dev(W.basisA); free_dev(W.basis64); free_dev(W.aho); free_dev(W.cdec);
         free_dev(W.bop); free_dev(W.csc); free_dev(W.th0); free_dev(W.lm); free_dev(W.m0max);
         const int nmh = std::max(W.split.nmh, 1);
-        if (hipMalloc((void **)&W.basisA, sizeof(double) * (size_t)(ngroups * W.NI * 64)) != hipSuccess ||
-            hipMalloc((void **)&W.basis64, sizeof(double) * (size_t)(ngroups * 64)) != hipSuccess ||
-            hipMalloc((void **)&W.aho, sizeof(half8_t) * (size_t)(ngroups * nmh * 64)) != hipSuccess ||
-            hipMalloc((void **)&W.cdec, sizeof(float) * (size_t)(ngroups * 16)) != hipSuccess ||
-            hipMalloc((void **)&W.bop, sizeof(half8_t) * (size_t)(ngroups * NCB * 2 * 64)) != hipSuccess ||
-            hipMalloc((void **)&W.csc, sizeof(double) * (size_t)(8 * NCB)) != hipSuccess ||
-            hipMalloc((void **)&W.th0, sizeof(double) * (size_t)(ngroups * 16)) != hipSuccess ||
-            hipMalloc((void **)&W.lm, sizeof(float) * (size_t)(ngroups * 16)) != hipSuccess ||
-            hipMalloc((void **)&W.m0max, sizeof(float)) != hipSuccess) {
+        if (hom::dev_malloc((void **)&W.basisA, sizeof(double) * (size_t)(ngroups * W.NI * 64)) != hipSuccess ||
+            hom::dev_malloc((void **)&W.basis64, sizeof(double) * (size_t)(ngroups * 64)) != hipSuccess ||
+            hom::dev_malloc((void **)&W.aho, sizeof(half8_t) * (size_t)(ngroups * nmh * 64)) != hipSuccess ||
+            hom::dev_malloc((void **)&W.cdec, sizeof(float) * (size_t)(ngroups * 16)) != hipSuccess ||
+            hom::dev_malloc((void **)&W.bop, sizeof(half8_t) * (size_t)(ngroups * NCB * 2 * 64)) != hipSuccess ||
+            hom::dev_malloc((void **)&W.csc, sizeof(double) * (size_t)(8 * NCB)) != hipSuccess ||
+            hom::dev_malloc((void **)&W.th0, sizeof(double) * (size_t)(ngroups * 16)) != hipSuccess ||
+            hom::dev_malloc((void **)&W.lm, sizeof(float) * (size_t)(ngroups * 16)) != hipSuccess ||
+            hom::dev_malloc((void **)&W.m0max, sizeof(float)) != hipSuccess) {
             W.basis_flags = -1;
             return set_error(p, HOM_ERR_MEMORY, "device allocation failed for mixed-path operands");
         }
@@ -662,7 +662,7 @@ int mixed_run(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *
         if (need > W.partial_cap) {
             free_dev(W.partials);
             W.partial_cap = 0;
-            if (hipMalloc((void **)&W.partials, sizeof(float) * (size_t)need) != hipSuccess) {
+            if (hom::dev_malloc((void **)&W.partials, sizeof(float) * (size_t)need) != hipSuccess) {
                 W.partials = nullptr;
                 return set_error(p, HOM_ERR_MEMORY, "device allocation failed for mixed partials");
             }
@@ -671,7 +671,7 @@ int mixed_run(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *
         if (need_seed > W.seed_cap) {
             free_dev(W.seed);
             W.seed_cap = 0;
-            if (hipMalloc((void **)&W.seed, sizeof(double) * (size_t)need_seed) != hipSuccess) {
+            if (hom::dev_malloc((void **)&W.seed, sizeof(double) * (size_t)need_seed) != hipSuccess) {
                 W.seed = nullptr;
                 return set_error(p, HOM_ERR_MEMORY, "device allocation failed for phase seeds");
             }
@@ -695,7 +695,7 @@ int mixed_run(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *
                     if (p->d.partials) (void)hipFree(p->d.partials);
                     p->d.partials = nullptr;
                     p->d.partial_capacity = 0;
-                    if (hipMalloc(&p->d.partials, (size_t)need_p * p->complex_size()) != hipSuccess)
+                    if (hom::dev_malloc(&p->d.partials, (size_t)need_p * p->complex_size()) != hipSuccess)
                         return set_error(p, HOM_ERR_MEMORY, "device allocation failed for coil partials");
                     p->d.partial_capacity = need_p;
                 }

@@ -1215,13 +1215,13 @@ static int hom_server_create_impl(hom_server **out, const hom_server_config *con
     };
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         return dev_fail("hipStreamCreate failed.");
-    if (hipMalloc((void **)&s->dev_base, (size_t)c.device_bytes) != hipSuccess)
+    if (hom::dev_malloc((void **)&s->dev_base, (size_t)c.device_bytes) != hipSuccess)
         return dev_fail("Device arena allocation failed.");
     if (hipHostMalloc((void **)&s->host_base, (size_t)c.host_bytes, hipHostMallocDefault) != hipSuccess)
         return dev_fail("Pinned host arena allocation failed.");
     if (hipHostMalloc((void **)&s->h_desc, sizeof(StepDesc), hipHostMallocDefault) != hipSuccess)
         return dev_fail("Pinned descriptor allocation failed.");
-    if (hipMalloc((void **)&s->d_desc, sizeof(StepDesc)) != hipSuccess)
+    if (hom::dev_malloc((void **)&s->d_desc, sizeof(StepDesc)) != hipSuccess)
         return dev_fail("Device descriptor allocation failed.");
     std::memset(s->h_desc, 0, sizeof(StepDesc));
     s->dev_arena.init(c.device_bytes);
@@ -1281,7 +1281,7 @@ static int hom_server_put_phantom_impl(hom_server *s, int64_t ns, const double *
     if (const hipError_t e = hipSetDevice(s->cfg.device); e != hipSuccess) return hip_fail(s, e, "hipSetDevice");
     auto ph = std::make_unique<ResPhantom>();
     ph->ns = ns;
-    if (hipMalloc((void **)&ph->d, sizeof(double) * 7 * (size_t)ns) != hipSuccess)
+    if (hom::dev_malloc((void **)&ph->d, sizeof(double) * 7 * (size_t)ns) != hipSuccess)
         return fail(s, HOM_ERR_MEMORY, "Device allocation failed for the phantom.");
     for (int i = 0; i < 7; ++i) {
         hipError_t e = hipMemcpy(ph->d + (size_t)i * ns, src[i], sizeof(double) * ns, hipMemcpyHostToDevice);
@@ -1324,7 +1324,7 @@ static int hom_server_put_csm_impl(hom_server *s, int64_t phantom, int64_t nc, c
     cs->host.resize((size_t)(2 * ns * nc));
     for (int64_t c = 0; c < nc; ++c)
         std::memcpy(&cs->host[(size_t)(2 * c * ns)], csm + 2 * c * ld, sizeof(double) * 2 * (size_t)ns);
-    if (hipMalloc((void **)&cs->d, sizeof(double2) * (size_t)(ns * nc)) != hipSuccess)
+    if (hom::dev_malloc((void **)&cs->d, sizeof(double2) * (size_t)(ns * nc)) != hipSuccess)
         return fail(s, HOM_ERR_MEMORY, "Device allocation failed for the CSM.");
     hipError_t e = hipMemcpy(cs->d, cs->host.data(), sizeof(double2) * (size_t)(ns * nc), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -1376,8 +1376,8 @@ static int hom_server_put_sequence_impl(hom_server *s, const hom_sequence_host *
     std::copy(seq->psi, seq->psi + n, h.begin() + 4 * n);
     std::copy(seq->b1, seq->b1 + 2 * n, h.begin() + 5 * n);
     std::copy(seq->dt, seq->dt + ni, h.begin() + 7 * n);
-    if (hipMalloc((void **)&q->d, sizeof(double) * h.size()) != hipSuccess ||
-        hipMalloc((void **)&q->dadc, (size_t)n) != hipSuccess) {
+    if (hom::dev_malloc((void **)&q->d, sizeof(double) * h.size()) != hipSuccess ||
+        hom::dev_malloc((void **)&q->dadc, (size_t)n) != hipSuccess) {
         if (q->d) (void)hipFree(q->d);
         return fail(s, HOM_ERR_MEMORY, "Device allocation failed for the sequence.");
     }

@@ -149,3 +149,21 @@ def test_fused_rf_adc_capacity_and_empty_phantom():
     with make_plan("f64", empty, seq0, 20, coil_combination="fused_tiled") as plan:
         sig = run_block(plan, "precession", 0, 20, m, 20, 3, sentinel=complex(7, -3))
     assert np.all(sig == 0)
+
+
+def test_failed_allocation_leaves_no_stale_error():
+    """A device allocation that fails (1 PiB) is reported as such and does not
+    surface later as a launch error of the next block."""
+    require_gpu()
+    from hom_amd.runtime import HomDeviceError
+    obj, seq, mxy0, mz0 = high_order_workload(nspins=300, nsteps=20, ncoils=3)
+    m = BlochHighOrder("1111")
+    ref, _, _ = oracle.precession(obj, seq, 0, 20, m.flags, GAMMA, mxy0, mz0)
+    for mode in ("gemm", "fused_tiled"):
+        print_case("failed-alloc", mode)
+        with make_plan("f64", obj, seq, 20, coil_combination=mode) as plan:
+            with pytest.raises(HomDeviceError):
+                plan.buffer(1 << 50)
+            plan.set_state(mxy0, mz0)
+            sig = run_block(plan, "precession", 0, 20, m, 20, 3)
+        assert_isapprox(sig, ref, 1e-11, 1e-11, mode)
