@@ -116,6 +116,7 @@ int block_common(hom_plan *p, int64_t first, int64_t last, int flags, void *out,
     }
     if (nadc > 0) {
         rc = ensure_partials(p, nadc);
+        rc = rc ? rc : ensure_state_cols(p, nadc);
         if (rc) return rc;
     }
     rc = check_hip(p, rf ? launch_plain_excitation(p, first, last, flags)
@@ -134,6 +135,17 @@ void free_ptr(void *&ptr) {
 }  // namespace
 
 namespace hom {
+
+int ensure_state_cols(hom_plan *p, int64_t n_adc) {
+    if (n_adc <= p->d.state_cols || p->ns == 0) return HOM_OK;
+    if (const int e = check_hip(p, hipStreamSynchronize(p->stream), "hipStreamSynchronize")) return e;
+    free_ptr(p->d.states);
+    p->d.state_cols = 0;
+    if (dev_malloc(&p->d.states, p->complex_size() * (size_t)p->ns * (size_t)n_adc) != hipSuccess)
+        return set_error(p, HOM_ERR_MEMORY, "device allocation failed for ADC states");
+    p->d.state_cols = n_adc;
+    return HOM_OK;
+}
 
 int set_error(hom_plan *p, int code, const std::string &msg) {
     if (p)
@@ -256,9 +268,14 @@ static int hom_plan_create_impl(hom_plan **out, int device, int precision, int64
     int rc = HOM_OK;
     rc |= ensure_alloc(p, &p->d.mxy, Cb * ns);
     rc |= ensure_alloc(p, &p->d.mz, R * ns);
-    // ADC states: every block's in the per-block path, only those acquired
-    // during RF when the fused kernel takes the precession blocks
-    rc |= ensure_alloc(p, &p->d.states, Cb * ns * (size_t)(p->fused() ? max_rf_adc : max_adc));
+    // ADC states: every block's in the per-block path (allocated here, as the
+    // reference's workspace does); only those acquired during RF when the
+    // fused kernel (f32/f64 fused modes) or the segment kernels (mixed) take
+    // the precession blocks.  A mixed plan grows the buffer on demand when a
+    // precession block is stepped through the per-block entry point.
+    const int64_t cols = (p->fused() || precision == HOM_MIXED) ? max_rf_adc : max_adc;
+    rc |= ensure_alloc(p, &p->d.states, Cb * ns * (size_t)cols);
+    if (!rc) p->d.state_cols = cols;
     if (rc) {
         std::string msg = p->err;
         hom_plan_destroy(p);

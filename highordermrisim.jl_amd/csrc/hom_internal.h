@@ -65,7 +65,8 @@ struct DeviceArrays {
          *df = nullptr, *psi = nullptr;
     uint8_t *adc = nullptr;
     void *mxy = nullptr, *mz = nullptr;   // state: Ns complex, Ns real
-    void *states = nullptr;               // Ns x max_adc complex (column stride Ns)
+    void *states = nullptr;               // Ns x state_cols complex (column stride Ns)
+    int64_t state_cols = 0;               // ADC samples per spin the state buffer holds
     void *partials = nullptr;             // coil reduction partials
     int64_t partial_capacity = 0;         // in complex elements
     unsigned long long *stats = nullptr;  // upload statistics (max(17, Nc) words)
@@ -149,6 +150,9 @@ int guarded_host(F &&f) {
     }
 }
 int check_hip(hom_plan *p, hipError_t e, const char *what);
+// Grows the ADC-state buffer to hold n_adc samples per spin (after the
+// plan's stream has finished with the old one).
+int ensure_state_cols(hom_plan *p, int64_t n_adc);
 // Makes the plan's device current and, with sync, waits for its stream (so
 // that an earlier asynchronous failure is reported here); returns a status.
 int enter_device(hom_plan *p, bool sync);

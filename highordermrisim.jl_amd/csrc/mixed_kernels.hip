@@ -686,6 +686,7 @@ int mixed_run(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *
             // RF blocks: f64 per-block excitation (+ reduction of RF ADC samples).
             if (s.nadc > p->max_adc)
                 return set_error(p, HOM_ERR_DIMENSION, "The block acquires more samples than max_adc.");
+            if ((rc = ensure_state_cols(p, s.nadc))) return rc;
             rc = check_hip(p, launch_plain_excitation(p, s.first, s.last, flags), "excitation kernel");
             if (rc) return rc;
             if (s.nadc) {

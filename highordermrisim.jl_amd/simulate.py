@@ -196,9 +196,20 @@ def _finalize(sig, state, seconds, setup: SimulationSetup):
                        "t_sim_parts": setup.t_sim_parts, "type_sim_parts": setup.parts[2].astype(bool),
                        "Nblocks": len(setup.parts[0]), "sim_time_sec": seconds,
                        "max_adc_per_block": setup.max_adc, "max_rf_adc_per_block": setup.max_rf_adc,
-                       "adc_state_buffer_samples": setup.max_adc})
+                       "adc_state_buffer_samples": _state_buffer_samples(setup)})
     return signal_to_raw_data(mat, setup.seqd, phantom_name=getattr(setup.obj, "name", ""),
                               sim_params=raw_params, gamma=float(params.get("gamma", GAMMA)))
+
+
+def _state_buffer_samples(setup: SimulationSetup) -> int:
+    """ADC samples per spin of the plan's state buffer as created: RF-block
+    ADC only when precession blocks keep their states on chip (mixed
+    precision, fused coil modes), every block's ADC otherwise
+    (HighOrderGPU.jl:37).  A mixed run with callbacks steps its precession
+    blocks one at a time and grows the buffer to the largest such block."""
+    p = setup.params
+    on_chip = p["precision"] == "mixed" or p["coil_combination"] in ("fused_warp", "fused_tiled")
+    return setup.max_rf_adc if on_chip else setup.max_adc
 
 
 def _make_plan(setup: SimulationSetup) -> Plan:

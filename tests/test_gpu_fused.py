@@ -167,3 +167,23 @@ def test_failed_allocation_leaves_no_stale_error():
             plan.set_state(mxy0, mz0)
             sig = run_block(plan, "precession", 0, 20, m, 20, 3)
         assert_isapprox(sig, ref, 1e-11, 1e-11, mode)
+
+
+def test_mixed_plan_state_buffer_is_rf_only():
+    """A mixed plan sizes its ADC-state buffer for RF-block ADC only (its
+    precession segments keep states in registers): 2^20 spins x 10^6 samples
+    per block would need 16 TiB.  Per-block stepping of a precession block on
+    a mixed plan (callbacks) grows the buffer on demand and keeps FP64 parity."""
+    require_gpu()
+    print_case("mixed-state-buffer")
+    with Plan("mixed", 1 << 20, 32, 4, 10 ** 6, max_rf_adc=0):
+        pass
+    obj, seq, mxy0, mz0 = high_order_workload(nspins=300, nsteps=40, ncoils=3)
+    m = BlochHighOrder("1111")
+    ref, rmxy, _ = oracle.precession(obj, seq, 0, 40, m.flags, GAMMA, mxy0, mz0)
+    with make_plan("mixed", obj, seq, 40, max_rf_adc=0) as plan:
+        plan.set_state(mxy0, mz0)
+        sig = run_block(plan, "precession", 0, 40, m, 40, 3)
+        mxy, _ = plan.get_state()
+    assert_isapprox(sig, ref, 1e-11, 1e-11, "per-block mixed plan")
+    assert_isapprox(mxy, rmxy, 1e-11, 1e-11, "Mxy")
