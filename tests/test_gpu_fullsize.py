@@ -10,7 +10,9 @@ any size:
 * additivity over spins: the signal of a phantom equals the sum of the
   signals of two halves split at a 4096-spin tile boundary (NRMSE <= 1e-6:
   the halves get their own power-of-two scales);
-* parity on a spin subsample against the f64 oracle (NRMSE <= 1e-5).
+* parity on a spin subsample against the f64 oracle (NRMSE <= 1e-5);
+* the public simulate() with its default whole-run blocks gives the same
+  signal as 32-node blocks (the mixed path fuses free precession anyway).
 """
 import numpy as np
 import pytest
@@ -53,6 +55,15 @@ def test_fullsize_properties(config, subsample):
     sig, again = run(obj, seq, parts, method, twice=True)
     np.testing.assert_array_equal(sig, again)
     assert np.isfinite(sig).all() and np.abs(sig).max() > 0
+
+    # the public entry point with its default whole-run blocks: each readout
+    # is one block of n_adc samples, and the plan must not size an ADC-state
+    # buffer for it (4M spins x 10000 samples would be 640 GB)
+    from hom_amd.simulate import simulate
+    mat = simulate(obj, seq, sim_params={"precision": "mixed", "return_type": "mat",
+                                         "sim_method": method})
+    e = nrmse(mat[:, :, 0], sig)
+    assert e <= 1e-12, f"simulate() with whole-readout blocks: NRMSE {e:.3e}"
 
     sig2, _ = run(scaled(obj, 2.0), seq, parts, method)
     np.testing.assert_array_equal(sig2, 2.0 * sig)
