@@ -263,6 +263,12 @@ static int hom_plan_create_impl(hom_plan **out, int device, int precision, int64
         delete p;
         return set_error(nullptr, HOM_ERR_DEVICE, "hipStreamCreate failed.");
     }
+    // this device's address of the allocation-free reduction's scratch array
+    if (scratch_setup(device) != hipSuccess) {
+        (void)hipStreamDestroy(p->stream);
+        delete p;
+        return set_error(nullptr, HOM_ERR_DEVICE, "hipGetSymbolAddress failed for the reduction scratch array.");
+    }
     const size_t R = p->real_size(), Cb = p->complex_size();
     const size_t ns = (size_t)n_spins;
     int rc = HOM_OK;
@@ -495,6 +501,26 @@ static int hom_reduce_signal_impl(hom_plan *p, const void *states, int64_t ld_st
     return check_hip(p, launch_coil_reduce(p, states, ld_states, n_adc, out, out_ld), "coil reduction");
 }
 
+static int hom_reduce_signal_noalloc_impl(hom_plan *p, const void *states, int64_t ld_states, int64_t n_adc,
+                                          void *out, int64_t out_ld) {
+    if (!p) return HOM_ERR_ARGUMENT;
+    if (!p->has_csm) return set_error(p, HOM_ERR_STATE, "The CSM has not been uploaded.");
+    if (n_adc < 0) return set_error(p, HOM_ERR_ARGUMENT, "The ADC count must be nonnegative.");
+    if (n_adc == 0) return HOM_OK;
+    if (!out) return set_error(p, HOM_ERR_ARGUMENT, "The output pointer is NULL.");
+    if (out_ld < n_adc) return set_error(p, HOM_ERR_DIMENSION, "The output leading dimension is too small.");
+    if (p->ns > 0 && (!states || ld_states < p->ns))
+        return set_error(p, HOM_ERR_DIMENSION, "States, CSM, and signal dimensions do not match.");
+    const int64_t elems = coil_partial_elements(p->ns, n_adc, p->nc);
+    if (elems > (int64_t)(HOM_REDUCE_SCRATCH_BYTES / p->complex_size()))
+        return set_error(p, HOM_ERR_DIMENSION,
+                         "The reduction needs more partials than the scratch array holds "
+                         "(ceil(Ns / 2048) * n_adc * Nc complex values within HOM_REDUCE_SCRATCH_BYTES).");
+    if (const int e = enter_device(p, false)) return e;
+    return check_hip(p, launch_coil_reduce_scratch(p, states, ld_states, n_adc, out, out_ld),
+                     "coil reduction (scratch)");
+}
+
 static int hom_run_impl(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *rf, int64_t np,
                         int flags) {
     if (!p) return HOM_ERR_ARGUMENT;
@@ -688,6 +714,13 @@ int hom_reduce_signal(hom_plan *p, const void *states, int64_t ld_states, int64_
                       int64_t out_ld) {
     return guarded(p, [&] {
         return hom_reduce_signal_impl(p, states, ld_states, n_adc, out, out_ld);
+    });
+}
+
+int hom_reduce_signal_noalloc(hom_plan *p, const void *states, int64_t ld_states, int64_t n_adc, void *out,
+                              int64_t out_ld) {
+    return guarded(p, [&] {
+        return hom_reduce_signal_noalloc_impl(p, states, ld_states, n_adc, out, out_ld);
     });
 }
 

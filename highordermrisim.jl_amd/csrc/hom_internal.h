@@ -165,6 +165,14 @@ hipError_t launch_plain_excitation(hom_plan *p, int64_t first, int64_t last, int
 // sig[k, c] = sum_s states[s, k] * csm[s, c] into out (column stride out_ld).
 hipError_t launch_coil_reduce(hom_plan *p, const void *states, int64_t ld_states, int64_t n_adc,
                               void *out, int64_t out_ld);
+// The same reduction with the stage-1 partials in the library's per-device
+// scratch array (HOM_REDUCE_SCRATCH_BYTES) and no kernel timers: no device
+// allocation, no HIP event creation.  scratch_setup (device current, once per
+// device) looks up that device's address of the array; the launcher returns
+// hipErrorNotInitialized when it has not run for the plan's device.
+hipError_t scratch_setup(int device);
+hipError_t launch_coil_reduce_scratch(hom_plan *p, const void *states, int64_t ld_states,
+                                      int64_t n_adc, void *out, int64_t out_ld);
 int64_t coil_partial_elements(int64_t ns, int64_t n_adc, int64_t nc);
 int64_t fused_partial_elements(int64_t ns, int64_t n_adc, int64_t nc);
 hipError_t launch_fused_precession(hom_plan *p, int64_t first, int64_t last, int flags,

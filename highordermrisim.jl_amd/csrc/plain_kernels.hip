@@ -13,6 +13,7 @@
 #include "hom_internal.h"
 
 #include <algorithm>
+#include <mutex>
 
 namespace hom {
 namespace {
@@ -631,9 +632,12 @@ hipError_t excitation_t(hom_plan *p, int64_t first, int64_t last, int flags) {
     return hipGetLastError();
 }
 
+// Stage-1 partials go to `partials` (coil_partial_elements(ns, nadc, nc)
+// complex values); `timed` records the plan's kernel timers, which create HIP
+// events on first use (the allocation-free entry point passes false).
 template <typename T>
 hipError_t reduce_t(hom_plan *p, const void *states, int64_t lds, int64_t nadc, void *out,
-                    int64_t ldo) {
+                    int64_t ldo, void *partials, bool timed) {
     const int64_t nc = p->nc;
     if (nadc == 0 || nc == 0) return hipSuccess;
     if (p->ns == 0) {
@@ -645,21 +649,32 @@ hipError_t reduce_t(hom_plan *p, const void *states, int64_t lds, int64_t nadc, 
     const int64_t nch = nadc * nc;
     const int64_t ktiles = (nadc + 31) / 32, ctiles = (nc + 31) / 32;
     {
-        ScopedTimer timer(p, HOM_K_COIL_PARTIALS);
+        ScopedTimer timer(timed ? p : nullptr, HOM_K_COIL_PARTIALS);
         hipLaunchKernelGGL((k_coil_partials<T>), dim3((unsigned)ntiles, (unsigned)(ktiles * ctiles)),
                            dim3(256), 0, p->stream, p->ns, nadc, nc, (const Cx<T> *)states, lds,
                            (const Cx<T> *)p->d.csm, p->csm_ld ? p->csm_ld : p->ns,
-                           (Cx<T> *)p->d.partials, nch);
+                           (Cx<T> *)partials, nch);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     {
-        ScopedTimer timer(p, HOM_K_COIL_FINALIZE);
+        ScopedTimer timer(timed ? p : nullptr, HOM_K_COIL_FINALIZE);
         hipLaunchKernelGGL((k_coil_finalize<T>), dim3((unsigned)nch), dim3(256), 0, p->stream,
-                           ntiles, nadc, (const Cx<T> *)p->d.partials, (Cx<T> *)out, ldo);
+                           ntiles, nadc, (const Cx<T> *)partials, (Cx<T> *)out, ldo);
     }
     return hipGetLastError();
 }
+
+// Library-owned stage-1 partials of the allocation-free reduction: one copy
+// per device, holding the last call's values until the process ends.  Host
+// code never passes this name to a kernel (in host code it is a stand-in, not
+// a device address); it uses the address that scratch_address() looked up with
+// that device current.
+static __device__ __align__(256) unsigned char scratch[HOM_REDUCE_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
 
 }  // namespace
 
@@ -693,8 +708,37 @@ hipError_t launch_plain_excitation(hom_plan *p, int64_t first, int64_t last, int
 
 hipError_t launch_coil_reduce(hom_plan *p, const void *states, int64_t ld_states, int64_t n_adc,
                               void *out, int64_t out_ld) {
-    return p->is_f32() ? reduce_t<float>(p, states, ld_states, n_adc, out, out_ld)
-                       : reduce_t<double>(p, states, ld_states, n_adc, out, out_ld);
+    return p->is_f32() ? reduce_t<float>(p, states, ld_states, n_adc, out, out_ld, p->d.partials, true)
+                       : reduce_t<double>(p, states, ld_states, n_adc, out, out_ld, p->d.partials, true);
+}
+
+hipError_t scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void *ptr = nullptr;
+    // the caller made `device` current: the address is that device's copy
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return e;
+    }
+    g_scratch_ptr[device] = ptr;
+    return hipSuccess;
+}
+
+void *scratch_address(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
+hipError_t launch_coil_reduce_scratch(hom_plan *p, const void *states, int64_t ld_states,
+                                      int64_t n_adc, void *out, int64_t out_ld) {
+    void *ptr = scratch_address(p->device);
+    if (!ptr) return hipErrorNotInitialized;
+    return p->is_f32() ? reduce_t<float>(p, states, ld_states, n_adc, out, out_ld, ptr, false)
+                       : reduce_t<double>(p, states, ld_states, n_adc, out, out_ld, ptr, false);
 }
 
 hipError_t launch_fill_complex(hom_plan *p, void *out, int64_t rows, int64_t cols, int64_t ld) {

@@ -22,6 +22,8 @@ from .types import GAMMA, BlochHighOrder, DimensionMismatch, HO_DiscreteSequence
 LIB_PATH = Path(os.environ.get("HOM_LIBRARY") or Path(__file__).resolve().parent / "libhom.so")
 
 HOM_OK, HOM_ERR_ARGUMENT, HOM_ERR_DIMENSION, HOM_ERR_DEVICE, HOM_ERR_MEMORY, HOM_ERR_STATE = range(6)
+# Size of the per-device scratch array of hom_reduce_signal_noalloc (hom_abi.h).
+HOM_REDUCE_SCRATCH_BYTES = 32 * 1024 * 1024
 PRECISIONS = {"f32": 0, "f64": 1, "mixed": 2}
 COIL_MODES = {"gemm": 0, "reduction": 1, "fused_warp": 2, "fused_tiled": 3}
 KERNELS = {"plain_precession": 0, "plain_excitation": 1, "coil_partials": 2, "coil_finalize": 3,
@@ -36,7 +38,7 @@ ABI_SYMBOLS = (
     "hom_signal_element_size", "hom_buffer_alloc", "hom_buffer_free", "hom_buffer_upload",
     "hom_buffer_download", "hom_plan_set_timing", "hom_plan_kernel_time", "hom_kspace",
     "hom_kspace_rf_events", "hom_discretize", "hom_discrete_copy", "hom_discrete_free",
-    "hom_plan_mixed_stats",
+    "hom_plan_mixed_stats", "hom_reduce_signal_noalloc",
 )
 
 # Every symbol declared in include/hom_serve.h.
@@ -131,6 +133,7 @@ def load_library():
         "hom_precession_block": (C.c_int, [P, C.c_int64, C.c_int64, C.c_int, _vp, C.c_int64]),
         "hom_excitation_block": (C.c_int, [P, C.c_int64, C.c_int64, C.c_int, _vp, C.c_int64]),
         "hom_reduce_signal": (C.c_int, [P, _vp, C.c_int64, C.c_int64, _vp, C.c_int64]),
+        "hom_reduce_signal_noalloc": (C.c_int, [P, _vp, C.c_int64, C.c_int64, _vp, C.c_int64]),
         "hom_run": (C.c_int, [P, _i64p, _i64p, _u8p, C.c_int64, C.c_int]),
         "hom_synchronize": (C.c_int, [P]),
         "hom_signal_rows": (C.c_int, [P, _i64p]),
@@ -350,6 +353,14 @@ class Plan:
         ptr = out.ptr + offset_elems * np.dtype(self.cdtype).itemsize
         self._check(self.lib.hom_reduce_signal(self.handle, states.ptr if states else None,
                                                int(ld_states), int(n_adc), ptr, int(ld)))
+
+    def reduce_signal_noalloc(self, states: DeviceBuffer, ld_states: int, n_adc: int,
+                              out: DeviceBuffer, ld: int, offset_elems: int = 0):
+        """hom_reduce_signal_noalloc: the same reduction with its partials in the
+        library's per-device scratch array (no device allocation)."""
+        ptr = out.ptr + offset_elems * np.dtype(self.cdtype).itemsize
+        self._check(self.lib.hom_reduce_signal_noalloc(self.handle, states.ptr if states else None,
+                                                       int(ld_states), int(n_adc), ptr, int(ld)))
 
     def run(self, parts, method):
         pf = np.ascontiguousarray(parts[0], np.int64)

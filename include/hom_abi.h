@@ -144,6 +144,22 @@ int hom_excitation_block(hom_plan *plan, int64_t first, int64_t last, int flags,
 int hom_reduce_signal(hom_plan *plan, const void *states_dev, int64_t ld_states, int64_t n_adc,
                       void *out_dev, int64_t out_ld);
 
+/* No counterpart in the reference.  hom_reduce_signal for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): the same kernels and the same
+ * result, bit for bit, with the stage-1 partials in a library-owned device
+ * array of HOM_REDUCE_SCRATCH_BYTES instead of a plan buffer.  Each device
+ * holds one such array from library load until the process ends; it is
+ * looked up when a plan is created on that device.  ceil(Ns / 2048) * n_adc *
+ * Nc complex values of the plan's type must fit (HOM_ERR_DIMENSION otherwise:
+ * at 1,048,576 spins and FP64, n_adc * Nc <= 4096).  The call allocates
+ * nothing, records no kernel timers and is stream-ordered.  Every call on one
+ * device shares the array, so calls that overlap on one device (two plans'
+ * streams, or two threads) race: serialize them. */
+#define HOM_REDUCE_SCRATCH_BYTES (32 * 1024 * 1024)
+int hom_reduce_signal_noalloc(hom_plan *plan, const void *states_dev, int64_t ld_states,
+                              int64_t n_adc, void *out_dev, int64_t out_ld);
+
 /* Whole block loop (reference src/simulation/SimulatorCore.jl:96-152) over
  * the given parts, from the current state, writing sum(ADC) x Nc rows into the
  * plan's signal buffer.  HOM_F32/HOM_F64 plans run the per-block kernels;
