@@ -108,6 +108,8 @@ int block_common(hom_plan *p, int64_t first, int64_t last, int flags, void *out,
         return set_error(p, HOM_ERR_ARGUMENT, "A block with ADC samples needs an output buffer.");
     if (nadc > 0 && ld < nadc)
         return set_error(p, HOM_ERR_DIMENSION, "The output leading dimension is smaller than the block's ADC count.");
+    if (!rf && p->precision == HOM_MIXED)
+        return mixed_precession_block(p, first, last, flags, out, ld);
     if (!rf && p->fused()) {
         if (nadc > 0 && (rc = ensure_partial_elements(p, fused_partial_elements(p->ns, nadc, p->nc))))
             return rc;

@@ -187,6 +187,11 @@ hipError_t launch_csm_colmax2(hom_plan *p, unsigned long long *out);
 // ---- mixed path (mixed_kernels.hip) ----
 int mixed_run(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *rf, int64_t np,
               int flags);
+// One free-precession block [first, last] of a mixed plan as its own segment
+// (states stay on chip; no ADC-state buffer), its Nadc x Nc signal written to
+// out (complex double, column stride ld).  The sequence tables are cached for
+// the last part list, so stepping block by block rebuilds them per block.
+int mixed_precession_block(hom_plan *p, int64_t first, int64_t last, int flags, void *out, int64_t ld);
 void mixed_free(hom_plan *p);
 
 }  // namespace hom

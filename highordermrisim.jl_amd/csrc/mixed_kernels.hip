@@ -557,8 +557,11 @@ void mixed_free(hom_plan *p) {
     p->mixed = nullptr;
 }
 
-int mixed_run(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *rf, int64_t np,
-              int flags) {
+namespace {
+// The block loop of the mixed path over parts [pf, pl] (rf marks excitation
+// parts), writing sum(ADC) x Nc complex double rows into sig (column stride ld).
+int mixed_exec(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *rf, int64_t np,
+               int flags, double2 *sig, int64_t ld) {
     if (!p->mixed) p->mixed = new MixedWorkspace();
     MixedWorkspace &W = *p->mixed;
     p->mixed_segments = p->mixed_fp64_segments = 0;
@@ -679,8 +682,6 @@ int mixed_run(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *
         }
     }
 
-    double2 *sig = (double2 *)p->signal;
-    const int64_t ld = std::max<int64_t>(p->signal_rows, 1);
     for (const Segment &s : W.segs) {
         if (s.rf) {
             // RF blocks: f64 per-block excitation (+ reduction of RF ADC samples).
@@ -782,6 +783,17 @@ int mixed_run(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *
         }
     }
     return HOM_OK;
+}
+}  // namespace
+
+int mixed_run(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *rf, int64_t np,
+              int flags) {
+    return mixed_exec(p, pf, pl, rf, np, flags, (double2 *)p->signal, std::max<int64_t>(p->signal_rows, 1));
+}
+
+int mixed_precession_block(hom_plan *p, int64_t first, int64_t last, int flags, void *out, int64_t ld) {
+    const uint8_t rf0 = 0;
+    return mixed_exec(p, &first, &last, &rf0, 1, flags, (double2 *)out, std::max<int64_t>(ld, 1));
 }
 
 }  // namespace hom

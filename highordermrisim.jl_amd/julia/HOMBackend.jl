@@ -45,7 +45,7 @@ hom_available() = isfile(libhom)
 
 last_error(plan::Ptr{Cvoid}) = unsafe_string(ccall((:hom_last_error, libhom), Cstring, (Ptr{Cvoid},), plan))
 
-# Status -> the reference's exception types (src/simulation/Simulate.jl:226-278 conventions).
+# Status -> the reference's exception types (src/simulation/Simulate.jl:70-135 conventions).
 function check(status::Cint, plan::Ptr{Cvoid})
     status == HOM_OK && return nothing
     msg = last_error(plan)
@@ -85,7 +85,7 @@ u8vec(x) = Vector{UInt8}(collect(x))
 
 Device-resident plan for one phantom and one sampled sequence: the same
 argument checks as HighOrderGPUWorkspace (HighOrderGPU.jl:19-35), plus the
-upload of the phantom, CSM and sequence (Simulate.jl:303-309).
+upload of the phantom, CSM and sequence (Simulate.jl:151-157).
 """
 function HOMWorkspace(obj, hoseqd, max_adc; precision="f64", coil_combination="gemm",
                       max_rf_adc=max_adc, device=0, gamma)
@@ -155,7 +155,7 @@ end
 """
     hom_simulate_shard(obj, hoseqd, params, plan; callbacks=(), gamma)
 
-Drop-in for the kernel branch of simulate_shard (Simulate.jl:290-345): one
+Drop-in for the kernel branch of simulate_shard (Simulate.jl:138-193): one
 device plan, the whole block loop through hom_run when there are no callbacks
 (mixed precision fuses precession blocks), per-block entry points with state
 round trips when callbacks must observe each block (SimulatorCore.jl:109-111).

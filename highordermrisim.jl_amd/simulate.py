@@ -1,7 +1,7 @@
 """The reference's public simulation surface over the C ABI.
 
-- :func:`simulate`             src/simulation/Simulate.jl:204-219 (one-shot run)
-- :func:`setup_simulation`     src/simulation/Simulate.jl:222-287 (validation + block plan)
+- :func:`simulate`             src/simulation/Simulate.jl:52-67 (one-shot run)
+- :func:`setup_simulation`     src/simulation/Simulate.jl:70-135 (validation + block plan)
 - :func:`prepare_simulation`   src/simulation/PreparedSimulation.jl:144-179 (resident plan)
 - :class:`PreparedSimulation`  src/simulation/PreparedSimulation.jl:2-287 (reuse, close)
 
@@ -60,7 +60,7 @@ class SimulationSetup:
 
 def setup_simulation(obj: HO_Phantom, seqd: HO_DiscreteSequence, sim_params=None, gpus=None):
     """Validation and block plan, in the reference's order
-    (src/simulation/Simulate.jl:222-287); errors are raised before any device work."""
+    (src/simulation/Simulate.jl:70-135); errors are raised before any device work."""
     sim_params = {} if sim_params is None else sim_params
     if not isinstance(sim_params, dict) or not all(isinstance(k, str) for k in sim_params):
         raise ValueError("sim_params must be a dictionary with string keys.")
@@ -123,7 +123,7 @@ def setup_simulation(obj: HO_Phantom, seqd: HO_DiscreteSequence, sim_params=None
 
 
 def _run_shard(plan: Plan, setup: SimulationSetup, callbacks=(), initial=None):
-    """simulate_shard (src/simulation/Simulate.jl:290-345) on an existing plan."""
+    """simulate_shard (src/simulation/Simulate.jl:138-193) on an existing plan."""
     params = setup.params
     method = params["sim_method"]
     if initial is None:
@@ -176,7 +176,7 @@ def _run_shard(plan: Plan, setup: SimulationSetup, callbacks=(), initial=None):
 
 
 def _finalize(sig, state, seconds, setup: SimulationSetup):
-    """finalize_simulation! (src/simulation/Simulate.jl:347-366).  ADC phase
+    """finalize_simulation! (src/simulation/Simulate.jl:195-214).  ADC phase
     compensation: a per-sample phase array may be given as sim_params["adc_phase"]
     (radians); without it the compensation is the identity (ADC phase 0)."""
     params = setup.params
@@ -206,7 +206,8 @@ def _state_buffer_samples(setup: SimulationSetup) -> int:
     ADC only when precession blocks keep their states on chip (mixed
     precision, fused coil modes), every block's ADC otherwise
     (HighOrderGPU.jl:37).  A mixed run with callbacks steps its precession
-    blocks one at a time and grows the buffer to the largest such block."""
+    blocks one at a time, each as its own on-chip segment, so the buffer
+    keeps that size."""
     p = setup.params
     on_chip = p["precision"] == "mixed" or p["coil_combination"] in ("fused_warp", "fused_tiled")
     return setup.max_rf_adc if on_chip else setup.max_adc

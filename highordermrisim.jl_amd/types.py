@@ -8,7 +8,7 @@ error behaviour (``ValueError`` stands in for Julia's ``ArgumentError``;
 - :class:`HO_Phantom`           src/datatypes/HO_Phantom.jl:47-71, 97-109
 - :class:`HO_DiscreteSequence`  src/datatypes/HO_DiscreteSequence.jl:31-80
 - :func:`get_sim_ranges`        block partition consumed by
-                                src/simulation/Simulate.jl:268-273
+                                src/simulation/Simulate.jl:113-116
 
 Arrays are numpy: real per-spin vectors are float64, the coil sensitivity map
 is a Fortran-ordered (column-major, like Julia) complex128 ``Ns x Nc`` matrix,
@@ -236,7 +236,7 @@ def get_sim_ranges(seqd: HO_DiscreteSequence, max_block_length=np.inf,
     0-based inclusive node ranges; adjacent ranges share a node.
 
     This restates the contract the reference relies on from
-    KomaMRICore.get_sim_ranges (call site src/simulation/Simulate.jl:268-271):
+    KomaMRICore.get_sim_ranges (call site src/simulation/Simulate.jl:113-116):
     the exact chunk boundaries inside a run are KomaMRICore's own and are not
     reproduced; for free precession a different partition changes only
     rounding, and RF nodes are applied one at a time either way."""

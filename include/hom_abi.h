@@ -42,7 +42,7 @@ enum hom_status {
     HOM_ERR_STATE = 5
 };
 
-/* "precision" of the reference's sim_params (Simulate.jl:242-243), plus the
+/* "precision" of the reference's sim_params (Simulate.jl:90-91, 149-152), plus the
  * mixed-precision fast path (FP64 phase + split-FP16 MFMA coil reduction). */
 enum hom_precision { HOM_F32 = 0, HOM_F64 = 1, HOM_MIXED = 2 };
 
@@ -54,7 +54,7 @@ enum hom_coil_mode {
     HOM_COIL_FUSED_TILED = 3
 };
 
-/* BlochHighOrder flags (reference src/simulation/BlochHighOrder.jl:155-162). */
+/* BlochHighOrder flags (reference src/simulation/BlochHighOrder.jl:21-43). */
 enum hom_flags {
     HOM_HO0 = 1 << 0,
     HOM_HO1 = 1 << 1,
@@ -104,14 +104,16 @@ const char *hom_last_error(const hom_plan *plan);
  * As in the reference, an f32/f64 plan in a fused coil mode sizes its
  * ADC-state buffer for max_rf_adc samples: its precession blocks run the
  * fused kernel (FusedCoilGPU.jl:70-133) and only excitation blocks store
- * states, each at most max_rf_adc of them. */
+ * states, each at most max_rf_adc of them.  A mixed plan does the same: its
+ * precession blocks, run whole (hom_run) or one at a time
+ * (hom_precession_block), keep their states on chip. */
 int hom_plan_create(hom_plan **plan, int device, int precision, int64_t n_spins, int64_t n_coils,
                     int64_t n_nodes, int64_t max_adc, int64_t max_rf_adc, double gamma,
                     int coil_combination);
 int hom_plan_destroy(hom_plan *plan);
 
 /* Device residency of HO_Phantom fields (reference src/datatypes/HO_Phantom.jl:47-71;
- * upload step src/simulation/Simulate.jl:303-309).  csm is Ns x Nc column-major
+ * upload step src/simulation/Simulate.jl:151-157).  csm is Ns x Nc column-major
  * complex double with column stride ld (>= Ns). */
 int hom_upload_phantom(hom_plan *plan, const double *x, const double *y, const double *z,
                        const double *rho, const double *T1, const double *T2, const double *dw);
@@ -130,8 +132,12 @@ int hom_get_state(hom_plan *plan, double *mxy, double *mz);
  * evolve the state over nodes [first, last] and write the block's
  * Nadc x Nc signal (overwrite, transpose CSM, no conjugation) into device
  * memory out_dev of the plan's signal type with column stride out_ld.
- * out_dev may be NULL when the block has no ADC sample.  Stream-ordered: the
- * call returns before the device finishes. */
+ * out_dev may be NULL when the block has no ADC sample.  f32/f64 plans:
+ * stream-ordered, the call returns before the device finishes.  Mixed plans
+ * run a precession block as one mixed segment (out_dev is complex double);
+ * the sequence tables are built on the host for that block, so the call may
+ * wait for the plan's stream and allocate device memory for the tables and
+ * the segment buffers (HOM_ERR_MEMORY on failure). */
 int hom_precession_block(hom_plan *plan, int64_t first, int64_t last, int flags, void *out_dev,
                          int64_t out_ld);
 int hom_excitation_block(hom_plan *plan, int64_t first, int64_t last, int flags, void *out_dev,

@@ -301,7 +301,7 @@ int hom_oracle_run(const hom_oracle_phantom *ph, const hom_oracle_sequence *seq,
         return run_partition(ph, seq, part_first, part_last, part_is_rf, n_parts, flags, gamma,
                              0, ns, mxy, mz, sig, ld_sig);
     /* Private (SigmaADC x Nc) buffer per contiguous spin partition, summed in
-     * partition order: Simulate.jl:302 (sig(..., Nthreads)) and :337. */
+     * partition order: Simulate.jl:150 (sig(..., Nthreads)) and :185. */
     const size_t per = (size_t)(2 * total * nc);
     double *priv = (double *)calloc(per * (size_t)n_threads + 1, sizeof(double));
     if (!priv) return -3;

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-/* Field-order / off-resonance flags (src/simulation/BlochHighOrder.jl:155-162). */
+/* Field-order / off-resonance flags (src/simulation/BlochHighOrder.jl:21-43). */
 enum {
     HOM_ORACLE_HO0 = 1 << 0,
     HOM_ORACLE_HO1 = 1 << 1,
@@ -72,12 +72,12 @@ int hom_oracle_excitation(const hom_oracle_phantom *ph, const hom_oracle_sequenc
                           int64_t s0, int64_t s1,
                           double *mxy, double *mz, double *sig, int64_t ld_sig);
 
-/* Whole block loop (SimulatorCore.jl:96-152 with Simulate.jl:290-345):
+/* Whole block loop (SimulatorCore.jl:96-152 with Simulate.jl:138-193):
  * runs every part in order, writing each block's ADC rows in sequence into
  * sig (SigmaADC x Nc, column stride ld_sig).  The state arrays hold the
  * initial state on entry and the final state on return.  n_threads > 1 splits
  * the spins into contiguous partitions with private signal buffers summed in
- * partition order afterwards (SimulatorCore.jl:23-35, Simulate.jl:337). */
+ * partition order afterwards (SimulatorCore.jl:23-35, Simulate.jl:185). */
 int hom_oracle_run(const hom_oracle_phantom *ph, const hom_oracle_sequence *seq,
                    const int64_t *part_first, const int64_t *part_last,
                    const uint8_t *part_is_rf, int64_t n_parts, int flags, double gamma,

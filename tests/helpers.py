@@ -46,7 +46,7 @@ def round_f32(obj, seq):
 
 
 def tol(precision):
-    """Reference tolerances (test/gpu/test_gpu_kernels.jl:113)."""
+    """Reference tolerances (test/gpu/test_gpu_kernels.jl:31)."""
     return 3e-5 if precision == "f32" else 1e-11
 
 

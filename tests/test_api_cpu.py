@@ -1,5 +1,5 @@
 """Validation of the public surface before any device work
-(src/simulation/Simulate.jl:222-287; test/gpu/test_gpu_compatibility.jl:202-223)."""
+(src/simulation/Simulate.jl:70-135; test/gpu/test_gpu_compatibility.jl:202-223)."""
 import numpy as np
 import pytest
 

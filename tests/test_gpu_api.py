@@ -172,3 +172,41 @@ def test_prepared_simulation_reuse_and_failures():
         plan.close()
     with pytest.raises(ValueError):
         plan.simulate()
+
+
+def test_mixed_per_block_stepping_runs_mixed_kernels():
+    """A mixed plan stepped block by block (callbacks, synchronization="block")
+    runs each free-precession block on the mixed segment kernels, not the FP64
+    per-block path: hom_plan_mixed_stats counts the block as a mixed segment,
+    and the ADC-state buffer is never grown to the block's sample count."""
+    require_gpu()
+    from hom_amd.runtime import Plan
+    from hom_amd.workloads import make_config
+    obj, seq, cfg = make_config(1, spin_subsample=500)
+    m = BlochHighOrder("1111")
+    parts = get_sim_ranges(seq, 10 ** 9, 10 ** 9)
+    ref, rmxy, rmz = oracle.run(obj, seq, parts, m.flags, GAMMA)
+    adc = [seq.block_adc(a, b) for a, b in zip(parts[0], parts[1])]
+    total = int(sum(adc))
+    print_case("mixed-per-block", len(obj), len(parts[0]))
+    with Plan("mixed", len(obj), obj.n_coils, seq.n_nodes, max(adc), max_rf_adc=0) as plan:
+        plan.upload_phantom(obj)
+        plan.upload_sequence(seq)
+        plan.reset_state()
+        buf = plan.buffer(16 * total * obj.n_coils)
+        row = 0
+        for b in range(len(parts[0])):
+            first, last, rf = int(parts[0][b]), int(parts[1][b]), bool(parts[2][b])
+            fn = plan.excitation_block if rf else plan.precession_block
+            fn(first, last, m, buf if adc[b] else None, total, offset_elems=row)
+            if not rf and adc[b]:
+                assert plan.mixed_stats() == (1, 0)
+            row += adc[b]
+        plan.synchronize()
+        flat = np.zeros(total * obj.n_coils, dtype=np.complex128)
+        buf.download(flat)
+        sig = flat.reshape((total, obj.n_coils), order="F")
+        xy, z = plan.get_state()
+    assert nrmse(sig, ref) <= 1e-5
+    assert_isapprox(xy, rmxy, 1e-11, 1e-11, "Mxy")
+    assert_isapprox(z, rmz, 1e-11, 1e-11, "Mz")

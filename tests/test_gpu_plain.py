@@ -1,9 +1,9 @@
 """GPU parity of the per-block drop-in kernels (f32 / f64) against the oracle.
 
 Mirrors the reference's CUDA suites:
-- test/gpu/test_gpu_kernels.jl:98-168   precession (16 masks, coils, ADC, dw), analytic phase
-- test/gpu/test_gpu_kernels.jl:172-191  excitation
-- test/gpu/test_gpu_kernels.jl:195-218  packed output into strided signal views
+- test/gpu/test_gpu_kernels.jl:16-85   precession (16 masks, coils, ADC, dw), analytic phase
+- test/gpu/test_gpu_kernels.jl:90-109  excitation
+- test/gpu/test_gpu_kernels.jl:113-138  packed output into strided signal views
 - test/gpu/test_gpu_reduction.jl        deterministic coil reduction, guards, cancellation
 - test/gpu/test_gpu_compatibility.jl:183-187  block-length invariance
 Tolerances are the reference's: f32 3e-5, f64 1e-11 (atol = rtol, norm-wise).

@@ -1,14 +1,15 @@
 """Pins the CPU f64 oracle with closed forms, brute force and invariants.
 
 Each check follows a reference test or a fact the mathematics fixes:
-- per-channel analytic phase          test/gpu/test_gpu_kernels.jl:136-167
+- per-channel analytic phase          test/gpu/test_gpu_kernels.jl:54-85
 - zeroth-order readout phase          test/unit/test_koma_compatibility.jl:118-130
 - RF ignores high-order fields        test/unit/test_koma_compatibility.jl:132-135
 - 90-degree pulse, transpose CSM      test/unit/test_koma_compatibility.jl:55-70,97
 - ideal coil and CSM linearity        test/unit/test_phantom_construction.jl:123-150
 - block-length invariance             test/gpu/test_gpu_compatibility.jl:183-187
 - mask semantics over all 16 masks    closed form of src/simulation/kernels/HighOrderGPU.jl:73-93
-- spinor update                       brute force: Rodrigues rotation of the 3-vector
+- spinor update                       brute force: Rodrigues rotation of the 3-vector,
+                                      with df, dw (both Delta-w_excitation values), Gy, Gz
 - trapezoid exactness, T1/T2, frame phase psi   closed forms
 """
 import numpy as np
@@ -23,7 +24,7 @@ TWO_PI = 2 * np.pi
 
 
 def basis(x, y, z):
-    """The reference test's own basis expression (test_gpu_kernels.jl:155-159)."""
+    """The reference test's own basis expression (test_gpu_kernels.jl:72-76)."""
     r2 = x ** 2 + y ** 2 + z ** 2
     return np.array([np.ones_like(x), x, y, z, x * y, z * y, 3 * z ** 2 - r2, x * z, x ** 2 - y ** 2,
                      3 * y * x ** 2 - y ** 3, x * z * y, (5 * z ** 2 - r2) * y, 5 * z ** 3 - 3 * z * r2,
@@ -218,6 +219,44 @@ def test_spinor_matches_rodrigues_rotation():
         v = rodrigues(m0, bvec / B, -TWO_PI * GAMMA * B * dt)
         np.testing.assert_allclose([mxy[0].real, mxy[0].imag, mz[0]], v, rtol=1e-12, atol=1e-12)
         np.testing.assert_allclose(sig[0, 0], v[0] + 1j * v[1], rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("dw_exc", [True, False])
+def test_spinor_off_resonance_terms_match_rodrigues(dw_exc):
+    """One RF interval with every term of the rotation axis nonzero: the axis is
+    (Re B1, Im B1, Gx x + Gy y + Gz z + [dw_exc] dw/(2 pi gamma) - df/gamma)
+    (BlochHighOrderSimulatitonMethod.jl:116-117, HighOrderGPU.jl:203-208).
+    The off-resonance terms are sized like B1, so a flipped sign, a missing
+    2 pi or an ignored Delta-w_excitation flag moves the axis by O(1)."""
+    rng = np.random.default_rng(11 + dw_exc)
+    flags = BlochHighOrder("1111", dw_exc, True).flags
+    for _ in range(20):
+        b1 = complex(*rng.normal(0, 5e-6, 2))
+        g = rng.normal(0, 2e-4, 3)
+        r = rng.uniform(-0.05, 0.05, 3)
+        df = rng.normal(0, 200.0)                   # Hz: df / gamma ~ 5e-6 T
+        dw = TWO_PI * rng.normal(0, 200.0)          # rad/s: dw / (2 pi gamma) ~ 5e-6 T
+        dt = rng.uniform(1e-6, 1e-4)
+        m0 = rng.normal(size=3)
+        t = np.array([0.0, dt])
+        seq = HO_DiscreteSequence(t=t, gx=[g[0], 0], gy=[g[1], 0], gz=[g[2], 0], b1=[b1, 0],
+                                  df=[df, 0], psi=[0, 0], adc=[False, True], excitation=[True],
+                                  h=rng.normal(0, 1e-3, (16, 2)))   # ignored during RF
+        obj = HO_Phantom(x=[r[0]], y=[r[1]], z=[r[2]], dw=[dw], T1=[np.inf], T2=[np.inf])
+        sig, mxy, mz = oracle.excitation(obj, seq, 0, 1, flags, GAMMA,
+                                         np.array([m0[0] + 1j * m0[1]]), np.array([m0[2]]))
+        bz = g @ r + (dw / (TWO_PI * GAMMA) if dw_exc else 0.0) - df / GAMMA
+        bvec = np.array([b1.real, b1.imag, bz])
+        B = np.linalg.norm(bvec)
+        v = rodrigues(m0, bvec / B, -TWO_PI * GAMMA * B * dt)
+        np.testing.assert_allclose([mxy[0].real, mxy[0].imag, mz[0]], v, rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(sig[0, 0], v[0] + 1j * v[1], rtol=1e-12, atol=1e-12)
+        # the same check fails for an axis with either term's sign flipped
+        for wrong in (g @ r + (dw / (TWO_PI * GAMMA) if dw_exc else 0.0) + df / GAMMA,
+                      g @ r - (dw / (TWO_PI * GAMMA) if dw_exc else -dw / (TWO_PI * GAMMA)) - df / GAMMA):
+            bw = np.array([b1.real, b1.imag, wrong])
+            vw = rodrigues(m0, bw / np.linalg.norm(bw), -TWO_PI * GAMMA * np.linalg.norm(bw) * dt)
+            assert np.max(np.abs(vw - v)) > 1e-9
 
 
 def test_excitation_frame_phase_and_relaxation():
