@@ -13,24 +13,35 @@
 // Consecutive precession blocks therefore fuse into one segment; only RF
 // blocks are sequential barriers.
 //
-// Phase arithmetic.  The first-order and off-resonance terms (x, y, z, dw)
-// carry almost all of the phase (tens of revolutions) and run in FP64 on the
-// FP64 matrix pipe, one v_mfma_f64_16x16x4 per 16 spins x 16 samples.  The up
-// to 12 higher-order terms are split at anchor blocks of 256 samples: their
-// value at the block's first sample (the anchor) is computed per spin in FP64
-// and seeds the FP64 accumulator together with arg(M0); only the small
-// offsets from the anchor go through split-FP16 MFMAs (hi*hi + lo*hi + hi*lo
-// stacked along K), with FP32 accumulation.  FP64 MFMAs do not co-issue with
-// VALU work on gfx950; FP16 MFMAs do, so this moves three quarters of the
-// phase work onto the pipe that overlaps.  When a segment's offsets would not
-// fit FP16, the segment falls back to the all-FP64 phase body (main64_body).
+// Phase arithmetic: FP64 anchors, FP32 offsets.
+//   * Higher orders (up to 12 terms) are anchored per spin every 256 samples:
+//     the seed (k_mix_seed, FP64) holds arg(M0)/2pi plus their value at the
+//     anchor block's first sample.
+//   * The first-order and off-resonance terms (x, y, z, dw) carry almost all
+//     of the phase (tens of revolutions).  They are anchored every 8 samples:
+//     at the reference sample 4 of each half tile (samples 0-7 and 8-15 of a
+//     16-sample tile) the FP64 phase seed + sum_j b_j Kr_j[ref] is one
+//     v_mfma_f64_16x16x4 per 16 spins x 16 references, i.e. per 16-spin group
+//     and workgroup sample block of 8 tiles, reduced to [0, 1) revolutions and
+//     kept in LDS as FP32.
+//   * Everything between anchors is an offset: higher orders from the
+//     256-sample anchor, first order from the half-tile reference (at most 4
+//     samples away).  All offsets of a 16 x 16 tile come out of one or two
+//     split-FP16 MFMAs with FP32 accumulation: higher orders as hi*hi + lo*hi
+//     + hi*lo, first order as the six products of a three-way split whose
+//     index sum is at most 2 (33-bit operands).
+//   Per tile the phase is then anchor + offset in FP32.  FP64 MFMAs do not
+//   co-issue with VALU work on gfx950 and hold the VALU for their 64 cycles;
+//   one per group and sample block instead of one per tile is 1/8 of that.
+//   When a segment's offsets would not fit FP16, the segment falls back to
+//   the all-FP64 phase body (main64_body).
 //
 // MFMA lane layouts (measured on gfx950): f16 16x16x32 -- A[l%16][8(l/16)+i],
 // B[8(l/16)+i][l%16], D[4(l/16)+v][l%16]; f64 16x16x4 -- A[l%16][l/16],
-// B[l/16][l%16], D[l/16 + 4v][l%16].  Lane l therefore holds FP64 phases of
-// spins g + 4v (g = l/16) at sample l%16; every spin-side array is stored in
-// that order, and FP16 A operands use row m <-> spin m/4 + 4(m%4) so both
-// outputs line up in registers.
+// B[l/16][l%16], D[l/16 + 4v][l%16].  Lane l of an FP64 output therefore
+// holds spins g + 4v (g = l/16) in column l%16; every spin-side array is
+// stored in that order, and FP16 A operands use row m <-> spin m/4 + 4(m%4)
+// so the FP16 outputs D[4g + v] line up with it.
 #pragma once
 
 #include "hom_internal.h"
@@ -53,10 +64,18 @@ constexpr int kOpScale = 14;
 // B = dKr_j * 2^(kHOOut - a_j); the FP32 product sum is offset * 2^kHOOut.
 constexpr int kHOA = 10;
 constexpr int kHOOut = 20;
-// |B| <= 2^9 with |A| <= 2^10 bounds every offset by 0.5 revolutions, so the
-// offset can be added to the reduced FP64 phase in FP32 (error <= 1e-7 rev).
-// Segments with larger offsets use the all-FP64 phase body.
+// |B| <= 2^9 with |A| <= 2^10 bounds every higher-order offset by 0.5
+// revolutions.  Segments with larger offsets use the all-FP64 phase body.
 constexpr float kHOBMax = 512.0f;
+// First-order offsets: A = b_j * 2^a_j with max |A| in (2^13, 2^14], B =
+// dKr_j * 2^(kHOOut - a_j) (the same product scale, one accumulator), split
+// three ways; |B| <= 2^8 bounds each term by 4 revolutions, which FP32 holds
+// to 2.4e-7 revolutions.
+constexpr int kFOA = 14;
+constexpr float kFOBMax = 256.0f;
+// First-order references per 16-sample tile (samples 4 and 12).
+constexpr int kRefsPerTile = 2;
+constexpr int kRefHalf = 16 / kRefsPerTile;
 #ifndef HOM_TILES
 #define HOM_TILES 2
 #endif
@@ -71,6 +90,16 @@ constexpr int kBlockTiles = kWaves * kTiles;   // 16-sample tiles per workgroup 
 constexpr int kAnchorTiles = 16;
 constexpr int kBlocksPerAnchor = kAnchorTiles / kBlockTiles;
 static_assert(kAnchorTiles % kBlockTiles == 0, "anchor blocks must hold whole sample blocks");
+// The first-order references of a sample block are the 16 columns of one
+// FP64 MFMA.
+static_assert(kBlockTiles * kRefsPerTile == 16, "one FP64 MFMA per group and sample block");
+
+// Sample (within the segment) whose first-order Kr anchors half tile h of
+// tile tau: sample 4 of the half, clamped to the segment's last sample.
+inline int64_t ref_sample(int64_t tau, int h, int64_t nadc) {
+    const int64_t r = tau * 16 + h * kRefHalf + kRefHalf / 2;
+    return r < nadc ? r : nadc - 1;
+}
 
 // Active spatial terms of a field-order mask, in packing order.
 struct TermList {
@@ -78,14 +107,28 @@ struct TermList {
     int code[16];  // 1..15: basis index j; kTermDW: off-resonance
 };
 
-// The same terms split into the FP64 group (x, y, z, dw) and higher orders.
+// The same terms split into the first-order group (x, y, z, dw: FP64
+// anchors, three-way split offsets) and higher orders (256-sample anchors,
+// two-way split offsets).  Offset K slots: [0, 3 nho) higher order (slot
+// kind * nho + term, kind 0..2 = A hi/B hi, A lo/B hi, A hi/B lo), then
+// [3 nho, 3 nho + 6 n1) first order (slot 3 nho + p * n1 + term, product p
+// of kFOParts).
 struct PhaseSplit {
-    int code64[4];   // 1, 2, 3, kTermDW or 0
+    int code64[4];   // 1, 2, 3, kTermDW or 0 (FP64 anchor MFMA: K = 4)
+    int n1;          // first-order terms with a nonzero code (3 or 4)
+    int code1[4];    // their codes
+    int aexp1[4];    // A-side power-of-two exponent per first-order term
     int nho;         // number of higher-order terms (0, 5, 7 or 12)
-    int nmh;         // FP16 MFMAs per tile for the offsets: ceil(3 nho / 32)
+    int nmh;         // FP16 MFMAs per tile for the offsets: ceil((3 nho + 6 n1) / 32)
     int codeho[12];
     int aexp[12];    // A-side power-of-two exponent per higher-order term
 };
+
+// Three-way split products: A part and B part (0 hi, 1 mid, 2 lo) of product
+// p; every pair with index sum <= 2.
+__host__ __device__ inline int fo_apart(int p) { return p == 2 || p == 4 ? 1 : p == 5 ? 2 : 0; }
+__host__ __device__ inline int fo_bpart(int p) { return p == 1 || p == 4 ? 1 : p == 3 ? 2 : 0; }
+constexpr int kFOParts = 6;
 
 struct Segment {
     bool rf = false;
@@ -94,7 +137,7 @@ struct Segment {
     int64_t nadc = 0, row0 = 0;    // ADC samples, first signal row
     int64_t tiles = 0, nsb = 0;    // 16-sample tiles, sample blocks
     int64_t nab = 0;               // anchor blocks
-    int64_t kr_off = 0, k64_off = 0, bho_off = 0, s_off = 0, kend_off = 0, anc_off = 0;
+    int64_t kr_off = 0, kref_off = 0, bho_off = 0, s_off = 0, kend_off = 0, anc_off = 0;
     double T = 0.0;                // duration
 };
 
@@ -108,7 +151,7 @@ struct SeqHost {
 
 // Host FP64 tables of one run, concatenated over its segments.
 struct TableSet {
-    std::vector<double> kr, k64, kr0, kend, anc;
+    std::vector<double> kr, kref, kr0, kend, anc;
     std::vector<half8_t> bho;
     std::vector<float> tk;
 };
@@ -158,6 +201,15 @@ __device__ inline void split16(double v, _Float16 &hi, _Float16 &lo) {
     lo = (_Float16)(float)(v - (double)hi);
 }
 
+// x ~= p[0] + p[1] + p[2], each part fp16 of what the previous ones leave
+// (the differences are exact in FP64).
+__device__ inline void split16x3(double v, _Float16 (&p)[3]) {
+    p[0] = (_Float16)opaque((float)v);
+    const double r1 = v - (double)p[0];
+    p[1] = (_Float16)opaque((float)r1);
+    p[2] = (_Float16)(float)(r1 - (double)p[1]);
+}
+
 // ---- spin-side operands (one element per call; idx ranges in comments) ----
 
 // idx < ngroups * NI * 64: FP64 A operand of all terms (all-FP64 fallback).
@@ -175,7 +227,9 @@ __device__ inline void basisA_elem(int64_t idx, int64_t ns, int NI, const TermLi
     basisA[idx] = v;
 }
 
-// idx < ngroups * 64: FP64 A operand of (x, y, z, dw).
+// idx < (ngroups + 1) * 64: FP64 A operand of (x, y, z, dw); the extra group
+// is zero padding (a workgroup stages the next group's operand with every
+// group record).
 __device__ inline void basis64_elem(int64_t idx, int64_t ns, const PhaseSplit &ps, const double *x,
                                     const double *y, const double *z, const double *dw,
                                     double *basis64) {
@@ -185,10 +239,12 @@ __device__ inline void basis64_elem(int64_t idx, int64_t ns, const PhaseSplit &p
     basis64[idx] = (s < ns && code) ? basis_term(code, x[s], y[s], z[s], dw[s]) : 0.0;
 }
 
-// idx < ngroups * nmh * 64: FP16 A operand of the offset MFMAs.  K slot q
-// (0 <= q < 3 nho) holds term q % nho; slots [0, nho) and [2 nho, 3 nho) use
-// the high half of the scaled basis value, slots [nho, 2 nho) the low half
-// (paired with hi / hi / lo halves of the offsets on the B side).
+// idx < ngroups * nmh * 64: FP16 A operand of the offset MFMAs.  Higher-order
+// K slot q (0 <= q < 3 nho) holds term q % nho: slots [0, nho) and
+// [2 nho, 3 nho) the high half of the scaled basis value, slots [nho, 2 nho)
+// the low half (paired with hi / hi / lo halves of the offsets on the B side).
+// First-order slot 3 nho + p n1 + term holds part fo_apart(p) of the
+// three-way split of the scaled basis value.
 __device__ inline void aho_elem(int64_t idx, int64_t ns, const PhaseSplit &ps, const double *x,
                                 const double *y, const double *z, const double *dw, half8_t *aho) {
     const int lane = (int)(idx % 64);
@@ -206,6 +262,12 @@ __device__ inline void aho_elem(int64_t idx, int64_t ns, const PhaseSplit &ps, c
             _Float16 hi, lo;
             split16(b, hi, lo);
             v = kind == 1 ? lo : hi;
+        } else if (s < ns && q < 3 * ps.nho + kFOParts * ps.n1) {
+            const int r = q - 3 * ps.nho, term = r % ps.n1, prod = r / ps.n1;
+            const double b = ldexp(basis_term(ps.code1[term], x[s], y[s], z[s], dw[s]), ps.aexp1[term]);
+            _Float16 parts[3];
+            split16x3(b, parts);
+            v = parts[fo_apart(prod)];
         }
         out[i] = v;
     }
@@ -398,7 +460,8 @@ __device__ inline float reduce_rev(double ph) { return (float)__builtin_amdgcn_f
 // Operands of one main-kernel pass: one segment chunk (tiles [tile_lo,
 // tile_hi) of the segment tables, sample blocks from sb_lo) and one range of
 // coil blocks [cb0, cb0 + NCB) of ncb_total.  seed holds anchor blocks from
-// ab_lo; partials hold the chunk's rows only.
+// ab_lo (with 16 values of padding after the last); basis64 has ngroups + 1
+// groups; partials hold the chunk's rows only.
 struct MainOps {
     int64_t ngroups;
     int gpt;
@@ -410,7 +473,7 @@ struct MainOps {
     const float *lm;
     const double *seed;
     const half8_t *bop;
-    const double *k64;
+    const double *kref;   // per tile: 2 references x (x, y, z, dw) Kr, FP64
     const half8_t *bho;
     const float *tk;
     float *partials;
@@ -457,9 +520,11 @@ __device__ inline void write_partials(const floatx4 (&acc)[kTiles][NCB], const W
 }
 
 // Source of one 16-byte chunk of a group record: chunk c of group grp is at
-// p + grp * stride.  Record layout: coil B operand (nbop chunks), higher-order
-// A operand (naho), FP64 basis (32), decay rate (4), log-magnitude (4), seed
-// (8).  Chunks at or beyond nrec are never loaded.
+// p + grp * stride.  Record layout: coil B operand (nbop chunks), offset A
+// operand (naho), decay rate (4), log-magnitude (4), then the next group's
+// FP64 first-order basis (32) and phase seed (8), from which one wave
+// computes the next group's first-order anchors.  Chunks at or beyond nrec
+// are never loaded.
 struct StageSrc {
     const unsigned char *p;
     int64_t stride;
@@ -474,32 +539,33 @@ __device__ inline StageSrc stage_src(int c, int nbop, int naho, int nrec, const 
                 (int64_t)o.ncb_total * 2 * 64 * 16};
     if ((c -= nbop) < naho)
         return {reinterpret_cast<const unsigned char *>(o.aho) + (int64_t)c * 16, (int64_t)NMH * 64 * 16};
-    if ((c -= naho) < 32)
-        return {reinterpret_cast<const unsigned char *>(o.basis64) + (int64_t)c * 16, 64 * 8};
-    if ((c -= 32) < 4) return {reinterpret_cast<const unsigned char *>(o.cdec) + (int64_t)c * 16, 16 * 4};
+    if ((c -= naho) < 4) return {reinterpret_cast<const unsigned char *>(o.cdec) + (int64_t)c * 16, 16 * 4};
     if ((c -= 4) < 4) return {reinterpret_cast<const unsigned char *>(o.lm) + (int64_t)c * 16, 16 * 4};
-    c -= 4;
-    return {reinterpret_cast<const unsigned char *>(seed_b) + (int64_t)c * 16, 16 * 8};
+    if ((c -= 4) < 32)  // group grp + 1
+        return {reinterpret_cast<const unsigned char *>(o.basis64) + 64 * 8 + (int64_t)c * 16, 64 * 8};
+    c -= 32;
+    return {reinterpret_cast<const unsigned char *>(seed_b) + 16 * 8 + (int64_t)c * 16, 16 * 8};
 }
 
 // LDS of the split-phase body: two group-record buffers (double buffer) and
-// one private offset-operand slice per wave.  The three are separate
-// __shared__ arrays so the compiler can tell that the LDS reads of one record
-// buffer do not alias the direct global-to-LDS copy into the other, and does
-// not wait for that copy before reading.
+// two first-order anchor buffers (16 references x 16 spins, FP32).  They are
+// separate __shared__ arrays so the compiler can tell that the LDS accesses
+// of one buffer do not alias the direct global-to-LDS copy into the other
+// record, and does not wait for that copy before reading.
 template <int NMH, int NCB>
 struct SplitLds {
     static constexpr int kBopChunks = NCB * 2 * 64;    // 16-byte chunks
     static constexpr int kAhoChunks = NMH * 64;
-    static constexpr int kRecChunks = kBopChunks + kAhoChunks + 32 + 4 + 4 + 8;
+    static constexpr int kTailChunks = 4 + 4;          // decay rate, log-magnitude
+    static constexpr int kRecChunks = kBopChunks + kAhoChunks + kTailChunks + 32 + 8;
     static constexpr int kRecBytes = kRecChunks * 16;
-    static constexpr int kBvBytes = kTiles * (NMH > 0 ? NMH : 1) * 64 * 16;  // per wave
+    static constexpr int kAncBytes = 16 * 16 * 4;
     static constexpr int kStage = (kRecChunks + 255) / 256;  // chunks per thread
 };
 
 struct SplitSmem {
     unsigned char *rec0, *rec1;  // group records, SplitLds::kRecBytes each
-    unsigned char *bv;           // kWaves slices of SplitLds::kBvBytes
+    float *anc0, *anc1;          // first-order anchors of the records' groups
 };
 
 typedef __attribute__((address_space(3))) void lds_void_t;
@@ -514,23 +580,29 @@ __device__ inline void copy16_to_lds(const unsigned char *src, unsigned char *ld
 // Wait until this wave's global loads, and with them its LDS copies, are done.
 __device__ inline void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // vmcnt(0)
 
-// Split-phase body (FP64 first order + FP16 higher-order offsets) for one
-// work item.  Workgroup = kWaves waves over one spin tile; wave w owns kTiles
-// consecutive 16-sample tiles and accumulates a 16-sample x (16 NCB)-column
-// FP32 tile per sample tile over every 16-spin group of the spin tile.
+// Split-phase body (FP64 anchors + split-FP16 offsets) for one work item.
+// Workgroup = kWaves waves over one spin tile; wave w owns kTiles consecutive
+// 16-sample tiles and accumulates a 16-sample x (16 NCB)-column FP32 tile per
+// sample tile over every 16-spin group of the spin tile.
 //
-// The spin-group operands (coil B operand, higher-order A operand, FP64
-// basis, decay rate, log-magnitude, phase seed) are the same for all waves:
-// the workgroup copies each group's record into LDS once, double-buffered,
-// with direct global-to-LDS loads issued before the compute of the previous
-// group and awaited after it, one barrier per group.  Groups go in pairs, the
-// even one from rec0 and the odd one from rec1.  Each wave keeps its own
-// per-tile FP16 offset operands in a private LDS slice.  Every wave takes
-// part in the copies and barriers, with or without tiles; the body ends on a
-// barrier, so a persistent caller may reuse the LDS at once.
+// The spin-group operands (coil B operand, offset A operand, decay rate,
+// log-magnitude) are the same for all waves: the workgroup copies each
+// group's record into LDS once, double-buffered, with direct global-to-LDS
+// loads issued before the compute of the previous group and awaited after it,
+// one barrier per group.  Groups go in pairs, the even one from rec0/anc0 and
+// the odd one from rec1/anc1.  A record also carries the next group's FP64
+// first-order basis and phase seed: while the workgroup computes group g, one
+// wave (rotating) turns them into group g + 1's first-order anchors for the
+// 16 half-tile references of the sample block (one FP64 MFMA) and writes them
+// to the other anchor buffer, which the barrier publishes.  The per-tile
+// offset B operands stay in registers for the whole work item: LDS
+// bandwidth, not LDS space, is what the group loop runs short of.  Every wave
+// takes part in the copies and barriers, with or without tiles; the body ends
+// on a barrier, so a persistent caller may reuse the LDS at once.
 template <int NMH, int NCB>
 __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem &it,
                                                 const SplitSmem &sm) {
+    static_assert(NMH >= 1, "the first-order offsets need at least one FP16 MFMA");
     using L = SplitLds<NMH, NCB>;
     // Opaque thread id: a persistent caller runs this body in a loop, and
     // lane-dependent addresses must not be hoisted out of it (they would stay
@@ -541,19 +613,26 @@ __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem
     const int w = tid >> 6;
     const bool active = it.valid;
 
-    uint4 *bvl = reinterpret_cast<uint4 *>(sm.bv + w * L::kBvBytes);
-    double kv[kTiles];
     float tkv[kTiles];
+    half8_t bvr[kTiles][NMH];
 #pragma unroll
     for (int t = 0; t < kTiles; ++t) {
         const int64_t tile = it.tile0 + t;
         const bool ok = active && tile < o.tile_hi;
-        kv[t] = ok ? o.k64[tile * 64 + lane] : 0.0;
         tkv[t] = ok ? o.tk[tile * 16 + (lane & 15)] : 0.0f;
 #pragma unroll
         for (int m = 0; m < NMH; ++m)
-            bvl[(t * NMH + m) * 64 + lane] =
-                ok ? reinterpret_cast<const uint4 *>(o.bho)[(tile * NMH + m) * 64 + lane] : uint4{0, 0, 0, 0};
+            bvr[t][m] = ok ? o.bho[(tile * NMH + m) * 64 + lane] : half8_t{};
+    }
+    // B operand of the anchor MFMA: lane l holds Kr of first-order term l/16
+    // at reference column l%16 (tile wtile0 + col/2 of the sample block, half
+    // col%2).
+    const int64_t wtile0 = it.tile0 - (int64_t)w * kTiles;
+    double kref;
+    {
+        const int col = lane & 15;
+        const int64_t tile = wtile0 + col / kRefsPerTile;
+        kref = tile < o.tile_hi ? o.kref[(wtile0 * kRefsPerTile + col) * 4 + (lane >> 4)] : 0.0;
     }
 
     // Each thread copies chunks tid, tid + 256 and tid + 512 of every group
@@ -571,58 +650,70 @@ __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem
         if (L::kStage > 2 && u2) copy16_to_lds(s2.p + grp * s2.stride, dst + wbase + 512 * 16);
     };
 
+    // First-order anchors of one group from its FP64 basis (A: spin l%16,
+    // term l/16) and seed (C: spins l/16 + 4v, as the output): lane l writes
+    // the four spins' anchors at reference column l%16, reduced to [0, 1).
+    auto anchors = [&](double ba, doublex4 sd, float *dst) {
+        const doublex4 d = __builtin_amdgcn_mfma_f64_16x16x4f64(ba, kref, sd, 0, 0, 0);
+        floatx4 a;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) a[v] = reduce_rev(d[v]);
+        reinterpret_cast<floatx4 *>(dst)[(lane & 15) * 4 + (lane >> 4)] = a;
+    };
+
     floatx4 acc[kTiles][NCB];
 #pragma unroll
     for (int t = 0; t < kTiles; ++t)
 #pragma unroll
         for (int c = 0; c < NCB; ++c) acc[t][c] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-    auto compute = [&](const unsigned char *rec) {
+    // Anchor column read by this lane for local tile t: the half of lane%16.
+    const int acol0 = (w * kTiles) * kRefsPerTile + (lane & 15) / kRefHalf;
+
+    auto compute = [&](const unsigned char *rec, const float *anc, float *anc_next, bool next) {
         const half8_t *rb = reinterpret_cast<const half8_t *>(rec);
+        const unsigned char *tail = rec + (L::kBopChunks + L::kAhoChunks) * 16;
+        if (next) {
+            // Two 16-byte reads: a 32-byte LDS read would make the compiler
+            // wait for the in-flight copy into the other record buffer.
+            const unsigned char *nx = tail + L::kTailChunks * 16;
+            const double ba = reinterpret_cast<const double *>(nx)[lane];
+            const double2 sd01 = reinterpret_cast<const double2 *>(nx + 512)[2 * (lane >> 4)];
+            const double2 sd23 = reinterpret_cast<const double2 *>(nx + 512)[2 * (lane >> 4) + 1];
+            anchors(ba, doublex4{sd01.x, sd01.y, sd23.x, sd23.y}, anc_next);
+        }
+        if (!active) return;  // a wave past the segment's last tile
         half8_t bh[NCB], bl[NCB];
 #pragma unroll
         for (int c = 0; c < NCB; ++c) {
             bh[c] = rb[(c * 2 + 0) * 64 + lane];
             bl[c] = rb[(c * 2 + 1) * 64 + lane];
         }
-        half8_t ah[NMH > 0 ? NMH : 1];
+        half8_t ah[NMH];
 #pragma unroll
         for (int m = 0; m < NMH; ++m) ah[m] = rb[L::kBopChunks + m * 64 + lane];
-        const unsigned char *tail = rec + (L::kBopChunks + L::kAhoChunks) * 16;
-        const double ba = reinterpret_cast<const double *>(tail)[lane];
-        const floatx4 cd = reinterpret_cast<const floatx4 *>(tail + 512)[lane >> 4];
-        const floatx4 lmv = reinterpret_cast<const floatx4 *>(tail + 576)[lane >> 4];
-        // Two 16-byte reads: a 32-byte LDS read would make the compiler wait
-        // for the in-flight copy into the other record buffer.
-        const double2 sd01 = reinterpret_cast<const double2 *>(tail + 640)[2 * (lane >> 4)];
-        const double2 sd23 = reinterpret_cast<const double2 *>(tail + 640)[2 * (lane >> 4) + 1];
-        const doublex4 sd = {sd01.x, sd01.y, sd23.x, sd23.y};
-        const half8_t *bv = reinterpret_cast<const half8_t *>(bvl);
-        // phase and offset MFMAs of every tile first, then states + coil GEMM
-        // per tile (measured 1.8 % faster than phase, states and GEMM tile by
-        // tile: the second tile's offsets no longer wait behind the first
+        const floatx4 cd = reinterpret_cast<const floatx4 *>(tail)[lane >> 4];
+        const floatx4 lmv = reinterpret_cast<const floatx4 *>(tail + 64)[lane >> 4];
+        // offset MFMAs of every tile first, then states + coil GEMM per tile
+        // (the second tile's offsets then do not wait behind the first
         // tile's coil MFMAs)
-        doublex4 pht[kTiles];
         floatx4 offt[kTiles];
-#pragma unroll
-        for (int t = 0; t < kTiles; ++t) pht[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ba, kv[t], sd, 0, 0, 0);
 #pragma unroll
         for (int t = 0; t < kTiles; ++t) {
             offt[t] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int m = 0; m < NMH; ++m)
-                offt[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[m], bv[(t * NMH + m) * 64 + lane],
-                                                                 offt[t], 0, 0, 0);
+                offt[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[m], bvr[t][m], offt[t], 0, 0, 0);
         }
 #pragma unroll
         for (int t = 0; t < kTiles; ++t) {
-            const doublex4 ph = pht[t];
+            const floatx4 an =
+                reinterpret_cast<const floatx4 *>(anc)[(acol0 + t * kRefsPerTile) * 4 + (lane >> 4)];
             const floatx4 off = offt[t];
             uint4 ahu, alu;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                float rf = reduce_rev(ph[v]);
-                if (NMH > 0) rf = fmaf(off[v], 0x1p-20f, rf);
+                const float rf = fmaf(off[v], 0x1p-20f, an[v]);
                 const StateSplit e = state_split(rf, cd[v], tkv[t], lmv[v]);
                 (&ahu.x)[v] = __builtin_bit_cast(unsigned int, e.hi);
                 (&alu.x)[v] = __builtin_bit_cast(unsigned int, e.lo);
@@ -641,16 +732,24 @@ __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem
     const int64_t g0 = it.stile * o.gpt;
     const int64_t g1 = std::min<int64_t>(g0 + o.gpt, o.ngroups);
     stage(g0, sm.rec0);
+    if (w == 0) {
+        // anchors of the first group (later groups' come with the records)
+        const double ba = o.basis64[g0 * 64 + lane];
+        const doublex4 sd = *reinterpret_cast<const doublex4 *>(seed_b + g0 * 16 + 4 * (lane >> 4));
+        anchors(ba, sd, sm.anc0);
+    }
     wait_vmem();
     __syncthreads();
+    // wave computing the anchors of group grp + 1 during group grp
+    auto next_wave = [&](int64_t grp) { return grp + 1 < g1 && w == (int)((grp + 1 - g0) & (kWaves - 1)); };
     for (int64_t grp = g0; grp < g1; grp += 2) {
         if (grp + 1 < g1) stage(grp + 1, sm.rec1);
-        if (active) compute(sm.rec0);
+        compute(sm.rec0, sm.anc0, sm.anc1, next_wave(grp));
         wait_vmem();
         __syncthreads();
         if (grp + 1 >= g1) break;
         if (grp + 2 < g1) stage(grp + 2, sm.rec0);
-        if (active) compute(sm.rec1);
+        compute(sm.rec1, sm.anc1, sm.anc0, next_wave(grp + 1));
         wait_vmem();
         __syncthreads();
     }

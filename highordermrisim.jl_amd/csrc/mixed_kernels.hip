@@ -3,7 +3,8 @@
 //
 // Cached per (phantom, CSM, mask):
 //   k_mix_basis64 / k_mix_basis  FP64 basis values in FP64-MFMA A-operand order
-//   k_mix_aho      split-FP16 higher-order basis, FP16-MFMA A-operand order
+//   k_mix_aho      split-FP16 basis of the phase offsets (two-way higher
+//                  order, three-way first order), FP16-MFMA A-operand order
 //   k_mix_cdec     -log2(e)/T2 per spin, in MFMA output-row order
 //   k_mix_bop      B operand of the coil GEMM: split-FP16 (hi, lo) of
 //                  csm / S_c (S_c a power-of-two column scale)
@@ -11,7 +12,10 @@
 //   k_mix_m0max    max |M0| over spins (power-of-two state scale S)
 //   k_mix_seg      per spin: arg(M0) in revolutions (FP64), log2(|M0|/S) + 14
 //   k_mix_seed     per (anchor block, spin): arg(M0)/2pi + higher-order anchor
-//   k_mix_main     per 16 spins x 16 samples: phase, fract, v_sin/v_cos
+//   k_mix_main     per 16-spin group and 128-sample block: first-order FP64
+//                  anchors at 16 half-tile references (one FP64 MFMA); per 16
+//                  spins x 16 samples: offsets (split-FP16 MFMA), anchor +
+//                  offset, v_sin/v_cos
 //                  (revolutions), v_exp for |M0| e^{-t/T2}, FP16 split of the
 //                  state, complex coil GEMM as three split-FP16 products on
 //                  v_mfma_f32_16x16x32_f16, FP32 accumulation over a spin tile
@@ -42,7 +46,7 @@ struct MixedWorkspace {
     int NI = 0, NCB = 0;
     int64_t ngroups = 0;
     double *basisA = nullptr;   // ngroups x NI x 64 (all-FP64 fallback)
-    double *basis64 = nullptr;  // ngroups x 64
+    double *basis64 = nullptr;  // (ngroups + 1) x 64
     half8_t *aho = nullptr;     // ngroups x nmh x 64
     float *cdec = nullptr;      // ngroups x 16, MFMA output-row order
     half8_t *bop = nullptr;     // ngroups x NCB x 2 x 64
@@ -50,10 +54,10 @@ struct MixedWorkspace {
     double *th0 = nullptr;      // ngroups x 16: arg(M0) / 2pi
     float *lm = nullptr;        // ngroups x 16: log2(|M0| / S) + kOpScale
     float *m0max = nullptr;     // max |M0| of the current segment
-    double *seed = nullptr;     // max_nab x ngroups x 16
+    double *seed = nullptr;     // max_nab x ngroups x 16 + 16 padding
     int64_t seed_cap = 0;
     double *kr = nullptr;       // fallback FP64 tables (all terms)
-    double *k64 = nullptr;      // FP64 tables (x, y, z, dw)
+    double *kref = nullptr;     // FP64 Kr of (x, y, z, dw) at the half-tile references
     half8_t *bho = nullptr;     // FP16 offset tables
     double *anc = nullptr;      // per segment, per anchor block: higher-order anchors
     float *tk = nullptr;        // per-sample time within segment
@@ -89,7 +93,23 @@ PhaseSplit make_split(int flags, const double *basis_max) {
         for (int j = 4; j <= 8; ++j) s.codeho[s.nho++] = j;
     if (flags & HOM_HO3)
         for (int j = 9; j <= 15; ++j) s.codeho[s.nho++] = j;
-    s.nmh = (3 * s.nho + 31) / 32;
+    s.n1 = 0;
+    for (int q = 0; q < 4; ++q) {
+        s.code1[q] = 0;
+        s.aexp1[q] = 0;
+    }
+    for (int q = 0; q < 4; ++q) {
+        if (!s.code64[q]) continue;
+        const int c = s.code64[q];
+        s.code1[s.n1] = c;
+        if (basis_max[c] > 0.0) {
+            int e = 0;
+            frexp(basis_max[c], &e);
+            s.aexp1[s.n1] = kFOA - e;  // max |A| in [2^13, 2^14)
+        }
+        ++s.n1;
+    }
+    s.nmh = (3 * s.nho + kFOParts * s.n1 + 31) / 32;
     for (int q = 0; q < 12; ++q) {
         s.aexp[q] = 0;
         if (q < s.nho && basis_max[s.codeho[q]] > 0.0) {
@@ -116,7 +136,7 @@ __global__ void k_mix_basis64(int64_t ns, int64_t ngroups, PhaseSplit ps,
                               const double *__restrict__ z, const double *__restrict__ dw,
                               double *__restrict__ basis64) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx < ngroups * 64) basis64_elem(idx, ns, ps, x, y, z, dw, basis64);
+    if (idx < (ngroups + 1) * 64) basis64_elem(idx, ns, ps, x, y, z, dw, basis64);
 }
 
 __global__ void k_mix_aho(int64_t ns, int64_t ngroups, PhaseSplit ps,
@@ -190,10 +210,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HOM_WAVES_P
     using L = SplitLds<NMH, NCB>;
     __shared__ __attribute__((aligned(32))) unsigned char rec0[L::kRecBytes];
     __shared__ __attribute__((aligned(32))) unsigned char rec1[L::kRecBytes];
-    __shared__ __attribute__((aligned(32))) unsigned char bv[kWaves * L::kBvBytes];
+    __shared__ __attribute__((aligned(32))) float anc0[L::kAncBytes / 4];
+    __shared__ __attribute__((aligned(32))) float anc1[L::kAncBytes / 4];
     const int64_t work = xcd_work();
     if (work >= n_work) return;  // whole workgroup
-    main_split_body<NMH, NCB>(o, make_item(o, work % nsb, work / nsb), SplitSmem{rec0, rec1, bv});
+    main_split_body<NMH, NCB>(o, make_item(o, work % nsb, work / nsb), SplitSmem{rec0, rec1, anc0, anc1});
 }
 
 template <int NI, int NCB>
@@ -261,7 +282,6 @@ hipError_t launch_main(int ncb, bool fallback, int nmh, int NI, hipStream_t st, 
     const dim3 grid((unsigned)nwork);
     if (!fallback) {
         switch (nmh) {
-            case 0: return launch_split_nmh<0>(ncb, grid, st, o, nsb, nwork);
             case 1: return launch_split_nmh<1>(ncb, grid, st, o, nsb, nwork);
             case 2: return launch_split_nmh<2>(ncb, grid, st, o, nsb, nwork);
             default: return hipErrorInvalidValue;
@@ -342,6 +362,16 @@ void host_split16(double v, _Float16 &hi, _Float16 &lo) {
     std::memcpy(&lo, &lb, 2);
 }
 
+// v ~= p[0] + p[1] + p[2], each part the correctly rounded fp16 of what the
+// previous ones leave (the differences are exact in FP64).
+void host_split16x3(double v, _Float16 (&p)[3]) {
+    for (int k = 0; k < 3; ++k) {
+        const uint16_t b = half_bits_rne(v);
+        std::memcpy(&p[k], &b, 2);
+        v -= half_bits_to_double(b);
+    }
+}
+
 template <typename T>
 void free_dev(T *&q) {
     if (q) (void)hipFree(q);
@@ -352,10 +382,13 @@ void free_dev(T *&q) {
 
 // Host FP64 integration of the coefficient waveforms per segment, written in
 // MFMA B-operand lane order:
-//   kr  (all-FP64 fallback): tile tau, step i, lane l -> term 4i + l/16 at sample l%16
-//   k64 (FP64 first order):  tile tau, lane l -> term l/16 of (x, y, z, dw) at sample l%16
-//   bho (FP16 offsets):      tile tau, MFMA m, lane l, element i -> K slot
-//                            32m + 8(l/16) + i at sample l%16
+//   kr   (all-FP64 fallback): tile tau, step i, lane l -> term 4i + l/16 at sample l%16
+//   kref (first-order anchors): tile tau, half h, term q of (x, y, z, dw) ->
+//        Kr_q at the half's reference sample (ref_sample)
+//   bho  (FP16 offsets):      tile tau, MFMA m, lane l, element i -> K slot
+//                             32m + 8(l/16) + i at sample l%16: higher orders
+//                             from the 256-sample anchor, first order from the
+//                             sample's half-tile reference
 //   anc: per anchor block, the higher-order Kr at the block's first sample.
 // Runs fill(k0, k1, max) over [0, n) in up to 8 host threads (chunks of at
 // least 2048 items) and returns the largest of the threads' maxima.
@@ -387,7 +420,7 @@ void build_tables(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, i
                   std::vector<Segment> &segs, TableSet &ts) {
     segs.clear();
     ts = TableSet{};
-    const int NMH = ps.nmh, NHO = ps.nho;
+    const int NMH = ps.nmh, NHO = ps.nho, N1 = ps.n1;
     const int64_t nn = sq.nn;
     const bool ho1 = flags & HOM_HO1, ho0 = flags & HOM_HO0;
     auto wave_of = [&](int code) -> const double * {
@@ -397,11 +430,14 @@ void build_tables(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, i
     };
     std::vector<const double *> wave(tl.n);
     for (int q = 0; q < tl.n; ++q) wave[q] = wave_of(tl.code[q]);
-    // positions of the FP64 and higher-order terms inside the full list
-    int pos64[4] = {-1, -1, -1, -1}, posho[12];
+    // positions of the FP64, first-order and higher-order terms inside the
+    // full list
+    int pos64[4] = {-1, -1, -1, -1}, pos1[4] = {-1, -1, -1, -1}, posho[12];
     for (int q = 0; q < tl.n; ++q) {
         for (int j = 0; j < 4; ++j)
             if (ps.code64[j] && tl.code[q] == ps.code64[j]) pos64[j] = q;
+        for (int j = 0; j < N1; ++j)
+            if (tl.code[q] == ps.code1[j]) pos1[j] = q;
         for (int j = 0; j < NHO; ++j)
             if (tl.code[q] == ps.codeho[j]) posho[j] = q;
     }
@@ -409,9 +445,10 @@ void build_tables(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, i
     const double g = sq.gamma;
     // B-side scale of each higher-order term: a power of two, so the product
     // equals ldexp(., kHOOut - aexp) exactly
-    double bscale[12];
+    double bscale[12], bscale1[4];
     for (int q = 0; q < NHO; ++q) bscale[q] = std::ldexp(1.0, kHOOut - ps.aexp[q]);
-    auto &kr = ts.kr, &k64 = ts.k64, &kr0 = ts.kr0, &kend = ts.kend, &anc = ts.anc;
+    for (int q = 0; q < N1; ++q) bscale1[q] = std::ldexp(1.0, kHOOut - ps.aexp1[q]);
+    auto &kr = ts.kr, &kref = ts.kref, &kr0 = ts.kr0, &kend = ts.kend, &anc = ts.anc;
     auto &bho = ts.bho;
     auto &tk = ts.tk;
     {
@@ -430,7 +467,7 @@ void build_tables(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, i
             nab += ((nadc + 15) / 16 + kAnchorTiles - 1) / kAnchorTiles;
             b = e + 1;
         }
-        k64.reserve((size_t)(tiles * 64));
+        kref.reserve((size_t)(tiles * kRefsPerTile * 4));
         bho.reserve((size_t)(tiles * NMH * 64));
         tk.reserve((size_t)(tiles * 16));
         kr0.reserve((size_t)(tiles * 16));
@@ -461,12 +498,12 @@ void build_tables(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, i
         s.tiles = (s.nadc + 15) / 16;
         s.nsb = (s.tiles + kBlockTiles - 1) / kBlockTiles;
         s.nab = (s.tiles + kAnchorTiles - 1) / kAnchorTiles;
-        s.k64_off = (int64_t)k64.size();
+        s.kref_off = (int64_t)kref.size();
         s.bho_off = (int64_t)bho.size();
         s.s_off = (int64_t)tk.size();
         s.kend_off = (int64_t)kend.size();
         s.anc_off = (int64_t)anc.size();
-        k64.resize(k64.size() + (size_t)(s.tiles * 64), 0.0);
+        kref.resize(kref.size() + (size_t)(s.tiles * kRefsPerTile * 4), 0.0);
         bho.resize(bho.size() + (size_t)(s.tiles * NMH * 64), half8_t{});
         tk.resize(tk.size() + (size_t)(s.tiles * 16), 0.0f);
         kr0.resize(kr0.size() + (size_t)(s.tiles * 16), 0.0);
@@ -495,22 +532,45 @@ void build_tables(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, i
         for (int64_t ab = 0; ab < s.nab; ++ab)
             for (int q = 0; q < NHO; ++q)
                 anc[(size_t)(s.anc_off + ab * NHO + q)] = KrS[(size_t)(ab * 16 * kAnchorTiles * nt + posho[q])];
+        // first-order anchors: the values at each half tile's reference sample
+        for (int64_t tau = 0; tau < s.tiles; ++tau)
+            for (int h = 0; h < kRefsPerTile; ++h) {
+                const int64_t r = ref_sample(tau, h, s.nadc);
+                for (int q = 0; q < 4; ++q)
+                    kref[(size_t)(s.kref_off + (tau * kRefsPerTile + h) * 4 + q)] =
+                        pos64[q] >= 0 ? KrS[(size_t)(r * nt + pos64[q])] : 0.0;
+            }
         // Phase 2 (samples independent, split over threads): FP64 and
         // split-FP16 table entries, and the largest scaled offset.
         _Float16 *const bho16 = reinterpret_cast<_Float16 *>(bho.data());
+        // The largest |B| is reported relative to its bound: higher orders
+        // |B| / kHOBMax, first order |B| / kFOBMax (the segment falls back to
+        // the all-FP64 body above 1).
         auto fill = [&](int64_t k0, int64_t k1, double &bm_out) {
             double bm = 0.0;  // local: the threads' results share a cache line
             for (int64_t kk = k0; kk < k1; ++kk) {
                 const double *kr_k = &KrS[(size_t)(kk * nt)];
                 const int64_t tau = kk / 16, j = kk % 16, ab = kk / (16 * kAnchorTiles);
-                for (int q = 0; q < 4; ++q)
-                    k64[(size_t)(s.k64_off + tau * 64 + 16 * q + j)] = pos64[q] >= 0 ? kr_k[pos64[q]] : 0.0;
+                const double *kr_ref = &KrS[(size_t)(ref_sample(tau, (int)(j / kRefHalf), s.nadc) * nt)];
+                // first-order slot 3 NHO + p N1 + term holds part fo_bpart(p)
+                // of the three-way split offset from the half-tile reference
+                for (int term = 0; term < N1; ++term) {
+                    const double bval = (kr_k[pos1[term]] - kr_ref[pos1[term]]) * bscale1[term];
+                    bm = std::max(bm, std::fabs(bval) / kFOBMax);
+                    _Float16 parts[3];
+                    host_split16x3(bval, parts);
+                    for (int prod = 0; prod < kFOParts; ++prod) {
+                        const int slot = 3 * NHO + prod * N1 + term, m = slot / 32, sl = slot % 32;
+                        bho16[(size_t)(s.bho_off + (tau * NMH + m) * 64 + 16 * (sl / 8) + j) * 8 + sl % 8] =
+                            parts[fo_bpart(prod)];
+                    }
+                }
                 // slots term, NHO + term, 2 NHO + term hold the B-side hi, hi,
                 // lo of the term's split offset (the A side holds hi, lo, hi)
                 for (int term = 0; term < NHO; ++term) {
                     const double bval =
                         (kr_k[posho[term]] - anc[(size_t)(s.anc_off + ab * NHO + term)]) * bscale[term];
-                    bm = std::max(bm, std::fabs(bval));
+                    bm = std::max(bm, std::fabs(bval) / kHOBMax);
                     _Float16 hi, lo;
                     host_split16(bval, hi, lo);
                     for (int kind = 0; kind < 3; ++kind) {
@@ -526,7 +586,7 @@ void build_tables(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, i
             bm_out = bm;
         };
         const double bmax = parallel_max(s.nadc, fill);
-        s.fallback = !(bmax <= kHOBMax);
+        s.fallback = !(bmax <= 1.0);
         if (s.fallback) {
             // All-FP64 table of every term, only for segments that need it.
             s.kr_off = (int64_t)kr.size();
@@ -551,7 +611,7 @@ void mixed_free(hom_plan *p) {
     (void)hipStreamSynchronize(p->stream);  // releases proceed regardless
     free_dev(W->basisA); free_dev(W->basis64); free_dev(W->aho); free_dev(W->cdec);
     free_dev(W->bop); free_dev(W->csc); free_dev(W->th0); free_dev(W->lm); free_dev(W->m0max);
-    free_dev(W->seed); free_dev(W->kr); free_dev(W->k64); free_dev(W->bho); free_dev(W->anc);
+    free_dev(W->seed); free_dev(W->kr); free_dev(W->kref); free_dev(W->bho); free_dev(W->anc);
     free_dev(W->tk); free_dev(W->kr0); free_dev(W->kend); free_dev(W->partials);
     delete W;
     p->mixed = nullptr;
@@ -585,7 +645,7 @@ int mixed_exec(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t 
         free_dev(W.bop); free_dev(W.csc); free_dev(W.th0); free_dev(W.lm); free_dev(W.m0max);
         const int nmh = std::max(W.split.nmh, 1);
         if (hom::dev_malloc((void **)&W.basisA, sizeof(double) * (size_t)(ngroups * W.NI * 64)) != hipSuccess ||
-            hom::dev_malloc((void **)&W.basis64, sizeof(double) * (size_t)(ngroups * 64)) != hipSuccess ||
+            hom::dev_malloc((void **)&W.basis64, sizeof(double) * (size_t)((ngroups + 1) * 64)) != hipSuccess ||
             hom::dev_malloc((void **)&W.aho, sizeof(half8_t) * (size_t)(ngroups * nmh * 64)) != hipSuccess ||
             hom::dev_malloc((void **)&W.cdec, sizeof(float) * (size_t)(ngroups * 16)) != hipSuccess ||
             hom::dev_malloc((void **)&W.bop, sizeof(half8_t) * (size_t)(ngroups * NCB * 2 * 64)) != hipSuccess ||
@@ -607,7 +667,7 @@ int mixed_exec(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t 
                            (const double *)W.csc, W.bop);
         hipLaunchKernelGGL(k_mix_basis, dim3(blocks_for(ngroups * W.NI * 64, 256)), dim3(256), 0,
                            p->stream, ns, ngroups, W.NI, W.terms, dx, dy, dz, ddw, W.basisA);
-        hipLaunchKernelGGL(k_mix_basis64, dim3(blocks_for(ngroups * 64, 256)), dim3(256), 0,
+        hipLaunchKernelGGL(k_mix_basis64, dim3(blocks_for((ngroups + 1) * 64, 256)), dim3(256), 0,
                            p->stream, ns, ngroups, W.split, dx, dy, dz, ddw, W.basis64);
         if (W.split.nmh > 0)
             hipLaunchKernelGGL(k_mix_aho, dim3(blocks_for(ngroups * W.split.nmh * 64, 256)), dim3(256),
@@ -641,7 +701,7 @@ int mixed_exec(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t 
         TableSet ts;
         build_tables(sq, W.terms, W.split, W.NI, mflags, pf, pl, rf, np, W.segs, ts);
         rc = dev_upload(p, &W.kr, ts.kr);
-        rc = rc ? rc : dev_upload(p, &W.k64, ts.k64);
+        rc = rc ? rc : dev_upload(p, &W.kref, ts.kref);
         rc = rc ? rc : dev_upload(p, &W.bho, ts.bho);
         rc = rc ? rc : dev_upload(p, &W.anc, ts.anc);
         rc = rc ? rc : dev_upload(p, &W.tk, ts.tk);
@@ -658,7 +718,7 @@ int mixed_exec(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t 
     for (const Segment &s : W.segs)
         if (!s.rf) {
             need = std::max<int64_t>(need, nstiles * s.tiles * 16 * 16 * NCB);
-            need_seed = std::max<int64_t>(need_seed, s.nab * ngroups * 16);
+            need_seed = std::max<int64_t>(need_seed, s.nab * ngroups * 16 + 16);
         }
     if (need > W.partial_cap || need_seed > W.seed_cap) {
         if (const int e = check_hip(p, hipStreamSynchronize(p->stream), "hipStreamSynchronize")) return e;
@@ -745,7 +805,7 @@ int mixed_exec(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t 
                 o.lm = W.lm;
                 o.seed = W.seed;
                 o.bop = W.bop;
-                o.k64 = W.k64 + s.k64_off;
+                o.kref = W.kref + s.kref_off;
                 o.bho = W.bho + s.bho_off;
                 o.tk = W.tk + s.s_off;
                 o.partials = W.partials;

@@ -39,9 +39,9 @@ namespace hom {
 namespace serve {
 
 constexpr int kMaxJobs = HOM_SERVE_MAX_ACTIVE;
-// run_class cases 0..11: split phase, nmh 0..2 x ncb 1..4; 12..27: all-FP64,
+// run_class cases 0..7: split phase, nmh 1..2 x ncb 1..4; 8..23: all-FP64,
 // NI 1..4 x ncb 1..4
-constexpr int kSplitClasses = 12;
+constexpr int kSplitClasses = 8;
 constexpr int kMaxMain = 4 * kMaxJobs;
 constexpr int kLightGrid = 1024;
 constexpr int64_t kAlign = 256;
@@ -178,13 +178,12 @@ __device__ __forceinline__ void run_class(int cls, const MainOps &o, int64_t sb,
     case C: main_split_body<NMH, NCB>(o, it, sm); break;
 #define HOM_FP64(C, NI, NCB) \
     case C: main64_body<NI, NCB>(o, it); break;
-        HOM_SPLIT(0, 0, 1) HOM_SPLIT(1, 0, 2) HOM_SPLIT(2, 0, 3) HOM_SPLIT(3, 0, 4)
-        HOM_SPLIT(4, 1, 1) HOM_SPLIT(5, 1, 2) HOM_SPLIT(6, 1, 3) HOM_SPLIT(7, 1, 4)
-        HOM_SPLIT(8, 2, 1) HOM_SPLIT(9, 2, 2) HOM_SPLIT(10, 2, 3) HOM_SPLIT(11, 2, 4)
-        HOM_FP64(12, 1, 1) HOM_FP64(13, 1, 2) HOM_FP64(14, 1, 3) HOM_FP64(15, 1, 4)
-        HOM_FP64(16, 2, 1) HOM_FP64(17, 2, 2) HOM_FP64(18, 2, 3) HOM_FP64(19, 2, 4)
-        HOM_FP64(20, 3, 1) HOM_FP64(21, 3, 2) HOM_FP64(22, 3, 3) HOM_FP64(23, 3, 4)
-        HOM_FP64(24, 4, 1) HOM_FP64(25, 4, 2) HOM_FP64(26, 4, 3) HOM_FP64(27, 4, 4)
+        HOM_SPLIT(0, 1, 1) HOM_SPLIT(1, 1, 2) HOM_SPLIT(2, 1, 3) HOM_SPLIT(3, 1, 4)
+        HOM_SPLIT(4, 2, 1) HOM_SPLIT(5, 2, 2) HOM_SPLIT(6, 2, 3) HOM_SPLIT(7, 2, 4)
+        HOM_FP64(8, 1, 1) HOM_FP64(9, 1, 2) HOM_FP64(10, 1, 3) HOM_FP64(11, 1, 4)
+        HOM_FP64(12, 2, 1) HOM_FP64(13, 2, 2) HOM_FP64(14, 2, 3) HOM_FP64(15, 2, 4)
+        HOM_FP64(16, 3, 1) HOM_FP64(17, 3, 2) HOM_FP64(18, 3, 3) HOM_FP64(19, 3, 4)
+        HOM_FP64(20, 4, 1) HOM_FP64(21, 4, 2) HOM_FP64(22, 4, 3) HOM_FP64(23, 4, 4)
 #undef HOM_SPLIT
 #undef HOM_FP64
         default: break;
@@ -199,8 +198,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HOM_WAVES_P
     using L = SplitLds<2, 4>;
     __shared__ __attribute__((aligned(32))) unsigned char rec0[L::kRecBytes];
     __shared__ __attribute__((aligned(32))) unsigned char rec1[L::kRecBytes];
-    __shared__ __attribute__((aligned(32))) unsigned char bv[kWaves * L::kBvBytes];
-    const SplitSmem sm{rec0, rec1, bv};
+    __shared__ __attribute__((aligned(32))) float anc0[L::kAncBytes / 4];
+    __shared__ __attribute__((aligned(32))) float anc1[L::kAncBytes / 4];
+    const SplitSmem sm{rec0, rec1, anc0, anc1};
     __shared__ int64_t s_item;
     const int xcd = blockIdx.x % 8;
     const int64_t total = sd->total_items;
@@ -229,7 +229,7 @@ __global__ void k_srv_prep(int64_t ns, int64_t nc, int64_t ngroups, int NCB, int
                            const double *dw, const double *T2, const double *rho, const double2 *csm,
                            int64_t ldc, const double *csc, half8_t *bop, double *basis64,
                            half8_t *aho, float *cdec, double *basisA, double2 *mxy, double *mz) {
-    const int64_t n_bop = ngroups * NCB * 64, n_b64 = ngroups * 64, n_aho = ngroups * ps.nmh * 64,
+    const int64_t n_bop = ngroups * NCB * 64, n_b64 = (ngroups + 1) * 64, n_aho = ngroups * ps.nmh * 64,
                   n_cd = ngroups * 16, n_ba = basisA ? ngroups * NI * 64 : 0;
     const int64_t total = n_bop + n_b64 + n_aho + n_cd + n_ba + ns;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
@@ -250,7 +250,7 @@ __global__ void k_srv_prep(int64_t ns, int64_t nc, int64_t ngroups, int NCB, int
     }
 }
 
-int split_class(int nmh, int ncb) { return nmh * 4 + (ncb - 1); }
+int split_class(int nmh, int ncb) { return (nmh - 1) * 4 + (ncb - 1); }
 int fp64_class(int NI, int ncb) { return kSplitClasses + (NI - 1) * 4 + (ncb - 1); }
 
 // --------------------------------------------------------------- host --
@@ -413,7 +413,7 @@ struct Job {
     TableSet ts;
     // device layout (offsets into the job's arena block)
     int64_t o_mxy = 0, o_mz = 0, o_b64 = 0, o_aho = 0, o_cdec = 0, o_bop = 0, o_csc = 0, o_th0 = 0,
-            o_lm = 0, o_m0 = 0, o_seed = 0, o_ba = -1, o_kr = -1, o_k64 = 0, o_bho = 0, o_anc = 0,
+            o_lm = 0, o_m0 = 0, o_seed = 0, o_ba = -1, o_kr = -1, o_kref = 0, o_bho = 0, o_anc = 0,
             o_tk = 0, o_kr0 = 0, o_kend = 0, o_part = 0, o_sig = 0, o_rfst = -1, o_rfpart = -1;
     int64_t rf_part_cap = 0;
     int64_t dev_bytes = 0, dev_off = -1, host_bytes = 0, host_off = -1;
@@ -628,7 +628,7 @@ int plan_job(hom_server *s, Job &j, int gpt) {
     Layout L;
     j.o_mxy = L.add(16 * j.ns);
     j.o_mz = L.add(8 * j.ns);
-    j.o_b64 = L.add(8 * ng * 64);
+    j.o_b64 = L.add(8 * (ng + 1) * 64);
     j.o_aho = L.add(16 * ng * nmh * 64);
     j.o_cdec = L.add(4 * ng * 16);
     j.o_bop = L.add(16 * ng * j.NCB * 2 * 64);
@@ -636,10 +636,10 @@ int plan_job(hom_server *s, Job &j, int gpt) {
     j.o_th0 = L.add(8 * ng * 16);
     j.o_lm = L.add(4 * ng * 16);
     j.o_m0 = L.add(4);
-    j.o_seed = L.add(8 * (j.chunk_sb / kBlocksPerAnchor) * ng * 16);
+    j.o_seed = L.add(8 * ((j.chunk_sb / kBlocksPerAnchor) * ng * 16 + 16));
     j.o_ba = j.any_fallback ? L.add(8 * ng * j.NI * 64) : -1;
     j.o_kr = j.any_fallback ? L.add(8 * (int64_t)j.ts.kr.size()) : -1;
-    j.o_k64 = L.add(8 * (int64_t)j.ts.k64.size());
+    j.o_kref = L.add(8 * (int64_t)j.ts.kref.size());
     j.o_bho = L.add(16 * (int64_t)j.ts.bho.size());
     j.o_anc = L.add(8 * (int64_t)j.ts.anc.size());
     j.o_tk = L.add(4 * (int64_t)j.ts.tk.size());
@@ -710,7 +710,7 @@ int prepare_job(hom_server *s, Job &j) {
     const TableSet &t = j.ts;
     if (e == hipSuccess) e = up(j.o_csc, j.csc.data(), 8 * j.csc.size());
     if (e == hipSuccess && j.o_kr >= 0) e = up(j.o_kr, t.kr.data(), 8 * t.kr.size());
-    if (e == hipSuccess) e = up(j.o_k64, t.k64.data(), 8 * t.k64.size());
+    if (e == hipSuccess) e = up(j.o_kref, t.kref.data(), 8 * t.kref.size());
     if (e == hipSuccess) e = up(j.o_bho, t.bho.data(), 16 * t.bho.size());
     if (e == hipSuccess) e = up(j.o_anc, t.anc.data(), 8 * t.anc.size());
     if (e == hipSuccess) e = up(j.o_tk, t.tk.data(), 4 * t.tk.size());
@@ -819,7 +819,7 @@ bool add_chunk(hom_server *s, Job &j, StepDesc &d, StepPlan &sp) {
         o.lm = j.at<float>(j.o_lm);
         o.seed = j.at<double>(j.o_seed);
         o.bop = j.at<half8_t>(j.o_bop);
-        o.k64 = j.at<double>(j.o_k64) + g.k64_off;
+        o.kref = j.at<double>(j.o_kref) + g.kref_off;
         o.bho = j.at<half8_t>(j.o_bho) + g.bho_off;
         o.tk = j.at<float>(j.o_tk) + g.s_off;
         o.partials = j.at<float>(j.o_part);
