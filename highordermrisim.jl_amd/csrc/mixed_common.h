@@ -18,14 +18,13 @@
 //     the seed (k_mix_seed, FP64) holds arg(M0)/2pi plus their value at the
 //     anchor block's first sample.
 //   * The first-order and off-resonance terms (x, y, z, dw) carry almost all
-//     of the phase (tens of revolutions).  They are anchored every 8 samples:
-//     at the reference sample 4 of each half tile (samples 0-7 and 8-15 of a
-//     16-sample tile) the FP64 phase seed + sum_j b_j Kr_j[ref] is one
-//     v_mfma_f64_16x16x4 per 16 spins x 16 references, i.e. per 16-spin group
-//     and workgroup sample block of 8 tiles, reduced to [0, 1) revolutions and
-//     kept in LDS as FP32.
+//     of the phase (tens of revolutions).  They are anchored every 16
+//     samples: at the reference sample 8 of each 16-sample tile the FP64
+//     phase seed + sum_j b_j Kr_j[ref] is one v_mfma_f64_16x16x4 per 16 spins
+//     x 16 references, i.e. per 16-spin group and workgroup sample block of
+//     16 tiles, reduced to [0, 1) revolutions and kept in LDS as FP32.
 //   * Everything between anchors is an offset: higher orders from the
-//     256-sample anchor, first order from the half-tile reference (at most 4
+//     256-sample anchor, first order from the tile's reference (at most 8
 //     samples away).  All offsets of a 16 x 16 tile come out of one or two
 //     split-FP16 MFMAs with FP32 accumulation: higher orders as hi*hi + lo*hi
 //     + hi*lo, first order as the six products of a three-way split whose
@@ -70,17 +69,24 @@ constexpr float kHOBMax = 512.0f;
 // First-order offsets: A = b_j * 2^a_j with max |A| in (2^13, 2^14], B =
 // dKr_j * 2^(kHOOut - a_j) (the same product scale, one accumulator), split
 // three ways; |B| <= 2^8 bounds each term by 4 revolutions, which FP32 holds
-// to 2.4e-7 revolutions.
+// to 2.4e-7 revolutions (configuration 2 reaches about 2 revolutions at
+// 30 mT/m, 0.1 m and 8 samples of 2 us).
 constexpr int kFOA = 14;
 constexpr float kFOBMax = 256.0f;
-// First-order references per 16-sample tile (samples 4 and 12).
-constexpr int kRefsPerTile = 2;
+// First-order references per 16-sample tile (sample 8).
+constexpr int kRefsPerTile = 1;
 constexpr int kRefHalf = 16 / kRefsPerTile;
+// Four 16-sample tiles per wave at two waves per SIMD (up to 256 VGPRs):
+// each group record and coil operand read from LDS then serves 64 samples
+// per wave.  Two tiles per wave at three waves per SIMD (168 VGPRs) ran 5 %
+// slower, and one tile per wave with eight waves per workgroup (four per
+// SIMD) 11 % slower: the group loop is bound by LDS bytes per spin-sample
+// more than by occupancy.
 #ifndef HOM_TILES
-#define HOM_TILES 2
+#define HOM_TILES 4
 #endif
 #ifndef HOM_WAVES_PER_EU
-#define HOM_WAVES_PER_EU 3
+#define HOM_WAVES_PER_EU 2
 #endif
 constexpr int kWaves = 4;                      // waves per workgroup
 constexpr int kTiles = HOM_TILES;              // 16-sample tiles per wave
@@ -94,8 +100,8 @@ static_assert(kAnchorTiles % kBlockTiles == 0, "anchor blocks must hold whole sa
 // FP64 MFMA.
 static_assert(kBlockTiles * kRefsPerTile == 16, "one FP64 MFMA per group and sample block");
 
-// Sample (within the segment) whose first-order Kr anchors half tile h of
-// tile tau: sample 4 of the half, clamped to the segment's last sample.
+// Sample (within the segment) whose first-order Kr anchors part h of tile
+// tau: the middle sample of the part, clamped to the segment's last sample.
 inline int64_t ref_sample(int64_t tau, int h, int64_t nadc) {
     const int64_t r = tau * 16 + h * kRefHalf + kRefHalf / 2;
     return r < nadc ? r : nadc - 1;
@@ -593,7 +599,7 @@ __device__ inline void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // v
 // the odd one from rec1/anc1.  A record also carries the next group's FP64
 // first-order basis and phase seed: while the workgroup computes group g, one
 // wave (rotating) turns them into group g + 1's first-order anchors for the
-// 16 half-tile references of the sample block (one FP64 MFMA) and writes them
+// 16 tile references of the sample block (one FP64 MFMA) and writes them
 // to the other anchor buffer, which the barrier publishes.  The per-tile
 // offset B operands stay in registers for the whole work item: LDS
 // bandwidth, not LDS space, is what the group loop runs short of.  Every wave

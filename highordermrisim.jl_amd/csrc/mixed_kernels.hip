@@ -12,8 +12,8 @@
 //   k_mix_m0max    max |M0| over spins (power-of-two state scale S)
 //   k_mix_seg      per spin: arg(M0) in revolutions (FP64), log2(|M0|/S) + 14
 //   k_mix_seed     per (anchor block, spin): arg(M0)/2pi + higher-order anchor
-//   k_mix_main     per 16-spin group and 128-sample block: first-order FP64
-//                  anchors at 16 half-tile references (one FP64 MFMA); per 16
+//   k_mix_main     per 16-spin group and 256-sample block: first-order FP64
+//                  anchors at 16 tile references (one FP64 MFMA); per 16
 //                  spins x 16 samples: offsets (split-FP16 MFMA), anchor +
 //                  offset, v_sin/v_cos
 //                  (revolutions), v_exp for |M0| e^{-t/T2}, FP16 split of the
@@ -57,7 +57,7 @@ struct MixedWorkspace {
     double *seed = nullptr;     // max_nab x ngroups x 16 + 16 padding
     int64_t seed_cap = 0;
     double *kr = nullptr;       // fallback FP64 tables (all terms)
-    double *kref = nullptr;     // FP64 Kr of (x, y, z, dw) at the half-tile references
+    double *kref = nullptr;     // FP64 Kr of (x, y, z, dw) at the tile references
     half8_t *bho = nullptr;     // FP16 offset tables
     double *anc = nullptr;      // per segment, per anchor block: higher-order anchors
     float *tk = nullptr;        // per-sample time within segment
@@ -388,7 +388,7 @@ void free_dev(T *&q) {
 //   bho  (FP16 offsets):      tile tau, MFMA m, lane l, element i -> K slot
 //                             32m + 8(l/16) + i at sample l%16: higher orders
 //                             from the 256-sample anchor, first order from the
-//                             sample's half-tile reference
+//                             sample's tile reference
 //   anc: per anchor block, the higher-order Kr at the block's first sample.
 // Runs fill(k0, k1, max) over [0, n) in up to 8 host threads (chunks of at
 // least 2048 items) and returns the largest of the threads' maxima.
@@ -532,7 +532,7 @@ void build_tables(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, i
         for (int64_t ab = 0; ab < s.nab; ++ab)
             for (int q = 0; q < NHO; ++q)
                 anc[(size_t)(s.anc_off + ab * NHO + q)] = KrS[(size_t)(ab * 16 * kAnchorTiles * nt + posho[q])];
-        // first-order anchors: the values at each half tile's reference sample
+        // first-order anchors: the values at each tile's reference sample
         for (int64_t tau = 0; tau < s.tiles; ++tau)
             for (int h = 0; h < kRefsPerTile; ++h) {
                 const int64_t r = ref_sample(tau, h, s.nadc);
@@ -553,7 +553,7 @@ void build_tables(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, i
                 const int64_t tau = kk / 16, j = kk % 16, ab = kk / (16 * kAnchorTiles);
                 const double *kr_ref = &KrS[(size_t)(ref_sample(tau, (int)(j / kRefHalf), s.nadc) * nt)];
                 // first-order slot 3 NHO + p N1 + term holds part fo_bpart(p)
-                // of the three-way split offset from the half-tile reference
+                // of the three-way split offset from the tile reference
                 for (int term = 0; term < N1; ++term) {
                     const double bval = (kr_k[pos1[term]] - kr_ref[pos1[term]]) * bscale1[term];
                     bm = std::max(bm, std::fabs(bval) / kFOBMax);
