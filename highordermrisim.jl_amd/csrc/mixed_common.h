@@ -711,8 +711,9 @@ __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem
             for (int m = 0; m < NMH; ++m)
                 offt[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[m], bvr[t][m], offt[t], 0, 0, 0);
         }
-#pragma unroll
-        for (int t = 0; t < kTiles; ++t) {
+        // FP16 halves of the states of tile t: phase = anchor + offset, then
+        // sin, cos and decay
+        auto states = [&](int t, half8_t &ahi, half8_t &alo) {
             const floatx4 an =
                 reinterpret_cast<const floatx4 *>(anc)[(acol0 + t * kRefsPerTile) * 4 + (lane >> 4)];
             const floatx4 off = offt[t];
@@ -724,13 +725,31 @@ __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem
                 (&ahu.x)[v] = __builtin_bit_cast(unsigned int, e.hi);
                 (&alu.x)[v] = __builtin_bit_cast(unsigned int, e.lo);
             }
-            const half8_t ahi = __builtin_bit_cast(half8_t, ahu);
-            const half8_t alo = __builtin_bit_cast(half8_t, alu);
+            ahi = __builtin_bit_cast(half8_t, ahu);
+            alo = __builtin_bit_cast(half8_t, alu);
+        };
+        // Software pipeline: the states of tile t + 1 are computed while the
+        // coil MFMAs of tile t issue, one MFMA per four VALU instructions
+        // (measured 2 % faster than tile by tile; a 1:2 interleave, and the
+        // pipeline without one, gained less).
+        half8_t sh[2], sl[2];
+        states(0, sh[0], sl[0]);
+#pragma unroll
+        for (int t = 0; t < kTiles; ++t) {
+            if (t + 1 < kTiles) states(t + 1, sh[(t + 1) & 1], sl[(t + 1) & 1]);
+            const half8_t ahi = sh[t & 1], alo = sl[t & 1];
 #pragma unroll
             for (int c = 0; c < NCB; ++c) {
                 acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bh[c], acc[t][c], 0, 0, 0);
                 acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bl[c], acc[t][c], 0, 0, 0);
                 acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo, bh[c], acc[t][c], 0, 0, 0);
+            }
+            if (t + 1 < kTiles) {
+#pragma unroll
+                for (int k = 0; k < 3 * NCB; ++k) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+                    __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);  // four VALU
+                }
             }
         }
     };
