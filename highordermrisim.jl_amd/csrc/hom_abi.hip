@@ -293,6 +293,37 @@ static int hom_plan_create_impl(hom_plan **out, int device, int precision, int64
     return HOM_OK;
 }
 
+int hom_check_workspace_options(int device, int precision, int coil_combination, int64_t groupsize,
+                                int64_t adc_tile, int64_t coil_tile) {
+    g_create_error.clear();
+    if (coil_combination < HOM_COIL_GEMM || coil_combination > HOM_COIL_FUSED_TILED)
+        return set_error(nullptr, HOM_ERR_ARGUMENT, "Unknown coil_combination.");
+    if (precision != HOM_F32 && precision != HOM_F64 && precision != HOM_MIXED)
+        return set_error(nullptr, HOM_ERR_ARGUMENT, "The precision must be f32, f64 or mixed.");
+    const bool fused = coil_combination == HOM_COIL_FUSED_WARP || coil_combination == HOM_COIL_FUSED_TILED;
+    if (!fused) return HOM_OK;
+    if (!(groupsize >= 32 && groupsize <= 1024 && groupsize % 32 == 0))
+        return set_error(nullptr, HOM_ERR_ARGUMENT, "CUDA fused groupsize must be a multiple of 32, at most 1024.");
+    const int64_t A = coil_combination == HOM_COIL_FUSED_WARP ? 1 : adc_tile;
+    const int64_t C = coil_combination == HOM_COIL_FUSED_WARP ? 1 : coil_tile;
+    if (!((A == 1 || A == 2 || A == 4 || A == 8) && (C == 1 || C == 2 || C == 4)))
+        return set_error(nullptr, HOM_ERR_ARGUMENT,
+                         "Fused ADC/coil tiles must be selected from 1/2/4/8 and 1/2/4.");
+    int64_t shared = 64 * 1024;
+    if (device >= 0) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) {
+            (void)hipGetLastError();
+            return set_error(nullptr, HOM_ERR_DEVICE, "hipDeviceGetAttribute failed.");
+        }
+        shared = v;
+    }
+    const int64_t csize = precision == HOM_F32 ? 8 : 16;
+    if (groupsize * A * csize > shared)
+        return set_error(nullptr, HOM_ERR_ARGUMENT, "The fused ADC tile exceeds the device shared-memory limit.");
+    return HOM_OK;
+}
+
 int hom_plan_destroy(hom_plan *p) {
     if (!p) return HOM_OK;
     (void)hipSetDevice(p->device);

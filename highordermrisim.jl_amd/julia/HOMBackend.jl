@@ -9,10 +9,16 @@
 #   run_high_order_excitation!(ws, ...)   -> hom_excitation_block     (HighOrderGPU.jl:223-233)
 #   run_sim_time_iter! without callbacks  -> hom_run                  (SimulatorCore.jl:96-152)
 #
-# sim_params keep their meaning: "precision" => "f32" | "f64" | "mixed",
-# "coil_combination", "return_type", "max_block_length", ...  Host arrays are
-# passed as plain pointers in Julia's own layouts (column-major, interleaved
-# Complex{Float64}); BitVector flags are widened to UInt8 first.
+# sim_params keep their meaning ("precision" => "f32" | "f64",
+# "coil_combination", "return_type", "max_block_length", ...).  On this
+# backend "f32" runs the mixed-precision kernels (FP64 phase anchors,
+# split-FP16 MFMA coil GEMM, FP64 signal; NRMSE ~1e-6 against the f64 CPU
+# path, inside the reference's f32 tolerance of 3e-5) and returns
+# Complex{Float32}; "hip_precision" => "f32" selects the literal f32 kernels
+# instead, and "mixed" (accepted once setup_simulation lets it through)
+# returns Complex{Float64}.  Host arrays are passed as plain pointers in
+# Julia's own layouts (column-major, interleaved Complex{Float64}); BitVector
+# flags are widened to UInt8 first.
 module HOMBackend
 
 export HOMWorkspace, hom_simulate_shard, hom_available, hom_get_kspace, HOMServer,
@@ -81,17 +87,28 @@ u8vec(x) = Vector{UInt8}(collect(x))
 
 """
     HOMWorkspace(obj, hoseqd, max_adc; precision="f64", coil_combination="gemm",
-                 max_rf_adc=max_adc, device=0, gamma=γ)
+                 max_rf_adc=max_adc, groupsize=256, adc_tile=8, coil_tile=1, device=0, gamma=γ)
 
-Device-resident plan for one phantom and one sampled sequence: the same
-argument checks as HighOrderGPUWorkspace (HighOrderGPU.jl:19-35), plus the
-upload of the phantom, CSM and sequence (Simulate.jl:151-157).
+Device-resident plan for one phantom and one sampled sequence: the argument
+checks of HighOrderGPUWorkspace (HighOrderGPU.jl:19-35, including the fused
+groupsize / tile / shared-memory checks through hom_check_workspace_options)
+before any device work, then the upload of the phantom, CSM and sequence
+(Simulate.jl:151-157).
 """
 function HOMWorkspace(obj, hoseqd, max_adc; precision="f64", coil_combination="gemm",
-                      max_rf_adc=max_adc, device=0, gamma)
+                      max_rf_adc=max_adc, groupsize=256, adc_tile=8, coil_tile=1, device=0, gamma)
+    max_adc isa Integer && max_adc >= 0 ||
+        throw(ArgumentError("The maximum ADC count must be a nonnegative integer."))
+    max_rf_adc isa Integer && 0 <= max_rf_adc <= max_adc ||
+        throw(ArgumentError("The RF ADC count must be an integer between zero and max_adc."))
     haskey(PRECISIONS, precision) || throw(ArgumentError("The precision must be f32, f64 or mixed."))
     haskey(COIL_MODES, coil_combination) ||
         throw(ArgumentError("Unknown coil_combination: $coil_combination."))
+    groupsize isa Integer && adc_tile isa Integer && coil_tile isa Integer ||
+        throw(ArgumentError("Fused ADC/coil tiles must be selected from 1/2/4/8 and 1/2/4."))
+    check(ccall((:hom_check_workspace_options, libhom), Cint,
+                (Cint, Cint, Cint, Int64, Int64, Int64), device, PRECISIONS[precision],
+                COIL_MODES[coil_combination], groupsize, adc_tile, coil_tile), C_NULL)
     seqd = hoseqd.seqd
     ns, nc, nn = length(obj), size(obj.csm, 2), length(seqd.t)
     handle = Ref{Ptr{Cvoid}}(C_NULL)
@@ -152,30 +169,46 @@ function get_state!(M, ws::HOMWorkspace)
     return M
 end
 
-"""
-    hom_simulate_shard(obj, hoseqd, params, plan; callbacks=(), gamma)
+# Kernel precision behind a requested sim_params["precision"] (see the
+# module header): "f32" -> mixed unless "hip_precision" => "f32".
+function kernel_precision(params)
+    requested = get(params, "precision", "f32")
+    requested == "f32" && return get(params, "hip_precision", "mixed") == "f32" ? "f32" : "mixed"
+    return requested
+end
 
-Drop-in for the kernel branch of simulate_shard (Simulate.jl:138-193): one
-device plan, the whole block loop through hom_run when there are no callbacks
+"""
+    hom_simulate_shard(obj, hoseqd, params, plan; callbacks=(), gamma, M)
+
+Drop-in for simulate_shard (Simulate.jl:138-193) on one MI355X: one device
+plan, the whole block loop through hom_run when there are no callbacks
 (mixed precision fuses precession blocks), per-block entry points with state
 round trips when callbacks must observe each block (SimulatorCore.jl:109-111).
-Returns (; signal, state) with the signal as (sum(ADC), Nc, 1).
+M is the initial state (initialize_spins_state).  Returns simulate_shard's
+(; signal, state, params, device, seconds): the signal as (sum(ADC), Nc, 1)
+of the requested precision's complex type, seconds the wall time of the
+block loop (Simulate.jl:181-185 times the same span).
 """
 function hom_simulate_shard(obj, hoseqd, params, plan; callbacks=(), gamma, M)
     (; parts, excitation_bool, adc_counts, max_adc, max_rf_adc) = plan
-    precision = get(params, "precision", "f64")
+    precision = kernel_precision(params)
     ws = HOMWorkspace(obj, hoseqd, max_adc; precision, max_rf_adc, gamma,
-                      coil_combination=get(params, "coil_combination", "gemm"))
+                      coil_combination=get(params, "coil_combination", "gemm"),
+                      device=something(get(params, "gpu_device", 0), 0))
+    dev = Ref{Ptr{Cvoid}}(C_NULL)
     try
         set_state!(ws, M)
         flags = method_flags(params["sim_method"])
         pf = Int64[first(p) - 1 for p in parts]           # 0-based node ranges
         pl = Int64[last(p) - 1 for p in parts]
         rf = UInt8.(excitation_bool)
-        T = precision == "f32" ? ComplexF32 : ComplexF64
+        T = precision == "f32" ? ComplexF32 : ComplexF64   # the plan's signal type
         nc = size(obj.csm, 2)
         total = sum(adc_counts)
-        if isempty(callbacks)
+        t0 = time_ns()
+        # the reference's fence policy (SimulatorCore.jl:107-111): per-block
+        # steps when callbacks must see each block or synchronization="block"
+        if isempty(callbacks) && get(params, "synchronization", "auto") != "block"
             check(ccall((:hom_run, libhom), Cint,
                 (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{UInt8}, Int64, Cint),
                 ws.handle, pf, pl, rf, length(pf), flags), ws.handle)
@@ -184,11 +217,11 @@ function hom_simulate_shard(obj, hoseqd, params, plan; callbacks=(), gamma, M)
                 ws.handle, sig, max(total, 1)), ws.handle)
         else
             sig = zeros(T, total, nc)
-            dev = Ref{Ptr{Cvoid}}(C_NULL)
             bytes = sizeof(T) * total * nc
             check(ccall((:hom_buffer_alloc, libhom), Cint, (Ptr{Cvoid}, Int64, Ptr{Ptr{Cvoid}}),
                 ws.handle, bytes, dev), ws.handle)
             row = 0
+            rfs = 0
             for (b, p) in enumerate(parts)
                 fn = excitation_bool[b] ? :hom_excitation_block : :hom_precession_block
                 out = adc_counts[b] == 0 ? C_NULL : dev[] + sizeof(T) * row
@@ -201,10 +234,11 @@ function hom_simulate_shard(obj, hoseqd, params, plan; callbacks=(), gamma, M)
                         ws.handle, pf[b], pl[b], flags, out, max(total, 1))
                 check(st, ws.handle)
                 row += adc_counts[b]
+                rfs += excitation_bool[b] ? 1 : 0
                 check(ccall((:hom_buffer_download, libhom), Cint,
                     (Ptr{Cvoid}, Ptr{T}, Ptr{Cvoid}, Int64), ws.handle, sig, dev[], bytes), ws.handle)
                 get_state!(M, ws)
-                progress = (; block=b, Nblocks=length(parts), samples=row + 1)
+                progress = (; block=b, Nblocks=length(parts), rfs, samples=row + 1)
                 for cb in callbacks
                     cb(progress, (; excitation_bool, parts), (; obj, seqd=hoseqd, sig, Xt=M), params)
                 end
@@ -212,8 +246,17 @@ function hom_simulate_shard(obj, hoseqd, params, plan; callbacks=(), gamma, M)
             end
         end
         get_state!(M, ws)
-        return (; signal=reshape(sig, total, nc, 1), state=M)
+        seconds = (time_ns() - t0) / 1e9
+        out = get(params, "precision", "f32") == "f32" ? ComplexF32.(sig) : sig
+        params["implementation"] = "kernel"
+        params["adc_state_buffer_samples"] =
+            (precision == "mixed" || get(params, "coil_combination", "gemm") in ("fused_warp", "fused_tiled")) ?
+            max_rf_adc : max_adc
+        return (; signal=reshape(out, total, nc, 1), state=M, params, device="MI355X (HIP)", seconds)
     finally
+        if dev[] != C_NULL
+            ccall((:hom_buffer_free, libhom), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ws.handle, dev[])
+        end
         close!(ws)
     end
 end

@@ -38,7 +38,7 @@ ABI_SYMBOLS = (
     "hom_signal_element_size", "hom_buffer_alloc", "hom_buffer_free", "hom_buffer_upload",
     "hom_buffer_download", "hom_plan_set_timing", "hom_plan_kernel_time", "hom_kspace",
     "hom_kspace_rf_events", "hom_discretize", "hom_discrete_copy", "hom_discrete_free",
-    "hom_plan_mixed_stats", "hom_reduce_signal_noalloc",
+    "hom_plan_mixed_stats", "hom_reduce_signal_noalloc", "hom_check_workspace_options",
 )
 
 # Every symbol declared in include/hom_serve.h.
@@ -124,6 +124,8 @@ def load_library():
         "hom_plan_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.c_int64, C.c_int64,
                                       C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_int]),
         "hom_plan_destroy": (C.c_int, [P]),
+        "hom_check_workspace_options": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int64,
+                                                  C.c_int64]),
         "hom_upload_phantom": (C.c_int, [P, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
         "hom_upload_csm": (C.c_int, [P, _dp, C.c_int64]),
         "hom_upload_sequence": (C.c_int, [P, C.POINTER(SequenceHost)]),
@@ -191,6 +193,25 @@ def device_count() -> int:
     return n.value
 
 
+def check_workspace_options(precision: str, coil_combination: str, groupsize: int = 256,
+                            adc_tile: int = 8, coil_tile: int = 1, device: int = -1):
+    """The HighOrderGPUWorkspace keyword checks (HighOrderGPU.jl:24-35) through
+    hom_check_workspace_options; raises ValueError (Julia ArgumentError)."""
+    if precision not in PRECISIONS:
+        raise ValueError("The precision must be f32, f64 or mixed.")
+    if coil_combination not in COIL_MODES:
+        raise ValueError(f"Unknown coil_combination: {coil_combination}.")
+    for v in (groupsize, adc_tile, coil_tile):
+        if not isinstance(v, (int, np.integer)) or isinstance(v, bool):
+            raise ValueError("groupsize, adc_tile and coil_tile must be integers.")
+    lib = load_library()
+    rc = lib.hom_check_workspace_options(int(device), PRECISIONS[precision],
+                                         COIL_MODES[coil_combination], int(groupsize),
+                                         int(adc_tile), int(coil_tile))
+    if rc != HOM_OK:
+        _raise(rc, lib.hom_last_error(None).decode())
+
+
 def _raise(code: int, msg: str):
     if code == HOM_ERR_ARGUMENT:
         raise ValueError(msg)
@@ -254,11 +275,21 @@ class Plan:
 
     def __init__(self, precision: str, n_spins: int, n_coils: int, n_nodes: int, max_adc: int,
                  max_rf_adc: int | None = None, gamma: float = GAMMA,
-                 coil_combination: str = "gemm", device: int = 0):
+                 coil_combination: str = "gemm", device: int = 0, groupsize: int = 256,
+                 adc_tile: int = 8, coil_tile: int = 1):
+        """HighOrderGPUWorkspace's arguments (HighOrderGPU.jl:16-35), checked as
+        there before the device is touched; groupsize, adc_tile and coil_tile
+        are validated for the fused modes and select nothing else here."""
         if precision not in PRECISIONS:
             raise ValueError("The precision must be f32, f64 or mixed.")
         if coil_combination not in COIL_MODES:
             raise ValueError(f"Unknown coil_combination: {coil_combination}.")
+        for name, v in (("max_adc", max_adc), ("max_rf_adc", max_rf_adc)):
+            if v is not None and (isinstance(v, bool) or not isinstance(v, (int, np.integer))):
+                raise ValueError("The maximum ADC count must be a nonnegative integer."
+                                 if name == "max_adc" else
+                                 "The RF ADC count must be an integer between zero and max_adc.")
+        check_workspace_options(precision, coil_combination, groupsize, adc_tile, coil_tile)
         self.lib = load_library()
         self.precision = precision
         self.n_spins, self.n_coils, self.n_nodes = int(n_spins), int(n_coils), int(n_nodes)

@@ -110,6 +110,19 @@ const char *hom_last_error(const hom_plan *plan);
 int hom_plan_create(hom_plan **plan, int device, int precision, int64_t n_spins, int64_t n_coils,
                     int64_t n_nodes, int64_t max_adc, int64_t max_rf_adc, double gamma,
                     int coil_combination);
+
+/* The remaining HighOrderGPUWorkspace keyword checks (reference
+ * src/simulation/kernels/HighOrderGPU.jl:24-35), before any allocation:
+ * in a fused coil mode, groupsize a multiple of 32 in [32, 1024]; the ADC and
+ * coil tiles (both 1 in "fused_warp") from {1, 2, 4, 8} and {1, 2, 4}; and
+ * groupsize * adc_tile complex values of the precision within the device's
+ * shared memory per block (device < 0: the 64 KiB a workgroup may always
+ * use).  HOM_ERR_ARGUMENT with the reference's message otherwise.  The
+ * knobs select CUDA launch shapes in the reference; this library's fused
+ * kernel has one shape (256 spins, 16-sample tiles), so they are validated
+ * and not otherwise used. */
+int hom_check_workspace_options(int device, int precision, int coil_combination, int64_t groupsize,
+                                int64_t adc_tile, int64_t coil_tile);
 int hom_plan_destroy(hom_plan *plan);
 
 /* Device residency of HO_Phantom fields (reference src/datatypes/HO_Phantom.jl:47-71;

@@ -48,6 +48,28 @@ def test_plan_create_without_device_or_bad_args_reports_error():
     assert lib is rt.load_library()
 
 
+def test_workspace_keyword_checks():
+    """HighOrderGPUWorkspace's keyword checks (HighOrderGPU.jl:24-35) as the
+    reference's own test drives them (test/gpu/test_gpu_fused_coils.jl:63-75):
+    raised before any device work, so they run without a GPU."""
+    with pytest.raises(ValueError, match="multiple of 32"):
+        rt.Plan("f64", 1, 1, 3, 32, coil_combination="fused_warp", groupsize=48)
+    for adc_tile, coil_tile in ((3, 1), (4.0, 1), (4, 0)):
+        with pytest.raises(ValueError):
+            rt.Plan("f64", 1, 1, 3, 32, coil_combination="fused_tiled", adc_tile=adc_tile,
+                    coil_tile=coil_tile)
+    # 1024 spins x 8 samples of Complex{Float64} exceed a workgroup's 64 KiB
+    with pytest.raises(ValueError, match="shared-memory"):
+        rt.Plan("f64", 1, 1, 3, 32, coil_combination="fused_tiled", groupsize=1024)
+    for max_rf_adc in (-1, 33, 2.5):
+        with pytest.raises(ValueError, match="RF ADC"):
+            rt.Plan("f64", 1, 1, 3, 32, coil_combination="fused_tiled", max_rf_adc=max_rf_adc)
+    # fused_warp fixes both tiles at 1; gemm ignores the knobs, as the reference does
+    rt.check_workspace_options("f64", "fused_warp", 1024, 3, 0)
+    rt.check_workspace_options("f32", "gemm", 48, 3, 0)
+    rt.check_workspace_options("f32", "fused_tiled", 1024, 8, 4)  # 64 KiB exactly
+
+
 def test_mask_consistency():
     for bits in range(16):
         name = format(bits, "04b")
