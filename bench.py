@@ -101,7 +101,9 @@ def main():
     ap.add_argument("--serve-budget", type=int, default=0,
                     help="spin-samples per request per step (0: server default)")
     ap.add_argument("--serve-no-graphs", action="store_true")
-    ap.add_argument("--serve-no-autotune", action="store_true")
+    ap.add_argument("--serve-autotune", action="store_true",
+                    help="re-probe the serving kernel's shape instead of the persisted tuning")
+    ap.add_argument("--serve-no-autotune", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -157,6 +159,12 @@ def main():
     fin_ms, fin_n = plan.kernel_time("mixed_finalize")
     prep_ms, prep_n = plan.kernel_time("mixed_prep")
     plan.set_timing(False)
+    from hom_amd import tuning
+    ncb, nmh = tuning.shape_class(nc, method.flags)
+    tuned_plan = {"source": "highordermrisim.jl_amd/tuning.json" if tuning.load() else None,
+                  "shape_class": {"ncb": ncb, "nmh": nmh}, "spin_tile": plan.spin_tile}
+    log(f"tuning: shape class ncb={ncb} nmh={nmh}, spin tile {plan.spin_tile} groups "
+        f"({tuned_plan['source'] or 'library default'})")
     value = spin_samples * args.steps / elapsed
     log(f"mixed: {value:.4e} spin-samples/s, {elapsed / args.steps * 1e3:.2f} ms/step; "
         f"main kernel {seg_ms / max(seg_n, 1):.3f} ms avg over {seg_n} launches")
@@ -260,6 +268,9 @@ def main():
         cdt = time.perf_counter() - a
         cpu = {"value": len(sub) * total_adc / cdt, "unit": "spin-samples/s", "cores": threads,
                "kind": "port",
+               "algorithm": "the f64 oracle: a per-spin fused restatement of the reference CPU "
+                            "path (one pass per spin over the nodes, coil sum folded in), not "
+                            "its broadcast temporaries + zgemm form, which it likely outruns",
                "sample": f"every {args.cpu_subsample}th spin ({len(sub)} spins) x all {total_adc} "
                          f"samples, {nc} coils, f64, {threads} threads, {cdt:.1f}s"}
         log(f"cpu oracle: {cpu['value']:.4e} spin-samples/s")
@@ -314,12 +325,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "mixed(f64 phase, fp16x3 mfma, f32 tile acc, f64 signal)",
+        "dtype": "mixed(f64 phase anchors, split-fp16 offsets and fp16x3 coil mfma, f32 tile acc, "
+                 "f64 signal)",
         "data": "synthetic",
         "config": {"workload": cfg.name, "spins": ns, "coils": nc,
                    "interleaves": cfg.n_interleaves, "samples_per_interleave": cfg.n_adc,
                    "mask": method.name, "precision": "mixed", "block_length": L},
         "roofline": roofline,
+        "tuning": tuned_plan,
         "end_to_end": end_to_end,
         "cpu_baseline": cpu,
         "plain_f64_dropin": plain,
@@ -336,6 +349,7 @@ def main():
 
 def serve_benchmark(args) -> dict:
     """Serving stream over resident inputs; returns latency and throughput."""
+    from hom_amd import tuning
     from hom_amd.serving import SimulationServer, request_stream, run_stream
     from hom_amd.workloads import CONFIG_SEEDS, serving_inputs
 
@@ -348,7 +362,7 @@ def serve_benchmark(args) -> dict:
                           use_graphs=not args.serve_no_graphs) as srv:
         ph, sq = srv.put_phantom(sobj), srv.put_sequence(sseq)
         tuned = None
-        if not args.serve_no_autotune:
+        if args.serve_autotune:
             cus = srv.config()["main_grid"] // 3
             gpt, grid, secs = srv.autotune(ph, sq, spins=(0, 1 << 18), coils=32, n_samples=20000,
                                            gpts=[128, 256, 512], grids=[2 * cus, 3 * cus, 4 * cus])
@@ -381,6 +395,8 @@ def serve_benchmark(args) -> dict:
         "step_budget": cfg["step_budget"], "gpt": cfg["gpt"], "main_grid": cfg["main_grid"],
         "use_graphs": bool(cfg["use_graphs"]),
         "autotune": tuned,
+        "shape_source": "probed at start-up" if tuned else
+                        ("highordermrisim.jl_amd/tuning.json" if tuning.server_shape() else "library default"),
         "inputs": f"resident {len(sobj)} spins x {sobj.n_coils} coils, RF + {sseq.adc.sum()}-sample "
                   "spiral; requests: spins log-U[1e3,1e6], samples U[1000,20000], "
                   "coils {1,8,16,32}, masks {1100,1110,1111}",

@@ -673,6 +673,20 @@ static int hom_plan_set_timing_impl(hom_plan *p, int enable) {
     return HOM_OK;
 }
 
+int hom_plan_set_spin_tile(hom_plan *p, int groups) {
+    if (!p) return HOM_ERR_ARGUMENT;
+    if (groups < 16 || groups > 4096 || (groups & (groups - 1)) != 0)
+        return set_error(p, HOM_ERR_ARGUMENT, "The spin tile must be a power of two between 16 and 4096 groups.");
+    p->spin_tile = groups;
+    return HOM_OK;
+}
+
+int hom_plan_get_spin_tile(const hom_plan *p, int *groups) {
+    if (!p || !groups) return HOM_ERR_ARGUMENT;
+    *groups = p->spin_tile;
+    return HOM_OK;
+}
+
 int hom_plan_mixed_stats(const hom_plan *p, int64_t *segments, int64_t *fp64_segments) {
     if (!p || !segments || !fp64_segments) return HOM_ERR_ARGUMENT;
     *segments = p->mixed_segments;

@@ -105,6 +105,8 @@ struct hom_plan {
     int64_t signal_rows = 0, signal_capacity_rows = 0;
 
     hom::MixedWorkspace *mixed = nullptr;
+    // 16-spin groups per spin tile of the mixed kernels (hom_plan_set_spin_tile)
+    int spin_tile = 256;
     // Last mixed run: precession segments with samples, and how many of them
     // ran the all-FP64 phase body.
     int64_t mixed_segments = 0, mixed_fp64_segments = 0;

@@ -30,10 +30,6 @@
 #include <system_error>
 #include <thread>
 
-#ifndef HOM_GPT
-#define HOM_GPT 256
-#endif
-
 namespace hom {
 
 struct MixedWorkspace {
@@ -66,7 +62,6 @@ struct MixedWorkspace {
     float *partials = nullptr;
     int64_t partial_cap = 0;
     std::vector<Segment> segs;
-    int gpt = HOM_GPT;  // 16-spin groups per spin tile
 };
 
 TermList make_terms(int flags) {
@@ -712,7 +707,8 @@ int mixed_exec(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t 
         W.table_parts = key;
     }
     // Partial and seed buffers sized for the largest segment.
-    const int64_t stile_spins = (int64_t)W.gpt * 16;
+    const int gpt = p->spin_tile;  // 16-spin groups per spin tile
+    const int64_t stile_spins = (int64_t)gpt * 16;
     const int64_t nstiles = std::max<int64_t>((ns + stile_spins - 1) / stile_spins, 1);
     int64_t need = 0, need_seed = 0;
     for (const Segment &s : W.segs)
@@ -792,7 +788,7 @@ int mixed_exec(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t 
                 const int ncb = std::min(4, NCB - cb0);
                 MainOps o{};
                 o.ngroups = ngroups;
-                o.gpt = W.gpt;
+                o.gpt = gpt;
                 o.cb0 = cb0;
                 o.ncb_total = NCB;
                 o.tile_lo = 0;

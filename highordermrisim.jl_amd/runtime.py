@@ -39,6 +39,7 @@ ABI_SYMBOLS = (
     "hom_buffer_download", "hom_plan_set_timing", "hom_plan_kernel_time", "hom_kspace",
     "hom_kspace_rf_events", "hom_discretize", "hom_discrete_copy", "hom_discrete_free",
     "hom_plan_mixed_stats", "hom_reduce_signal_noalloc", "hom_check_workspace_options",
+    "hom_plan_set_spin_tile", "hom_plan_get_spin_tile",
 )
 
 # Every symbol declared in include/hom_serve.h.
@@ -148,6 +149,8 @@ def load_library():
         "hom_plan_set_timing": (C.c_int, [P, C.c_int]),
         "hom_plan_kernel_time": (C.c_int, [P, C.c_int, _dp, _i64p]),
         "hom_plan_mixed_stats": (C.c_int, [P, _i64p, _i64p]),
+        "hom_plan_set_spin_tile": (C.c_int, [P, C.c_int]),
+        "hom_plan_get_spin_tile": (C.c_int, [P, C.POINTER(C.c_int)]),
         "hom_kspace": (C.c_int, [C.POINTER(SequenceHost), C.c_double, _u8p, C.c_int64, _dp, _dp,
                                  _dp, _dp]),
         "hom_kspace_rf_events": (C.c_int, [C.POINTER(SequenceHost), C.c_double, _i64p, _dp,
@@ -305,6 +308,8 @@ class Plan:
         self.handle = h.value
         self.cdtype = np.complex64 if precision == "f32" else np.complex128
         self._keep = None
+        self._tile_explicit = False
+        self._tile_applied = None
 
     # -- lifecycle ---------------------------------------------------------
     def close(self):
@@ -369,6 +374,7 @@ class Plan:
 
     def precession_block(self, first, last, method, out: DeviceBuffer | None, ld: int,
                          offset_elems: int = 0):
+        self._apply_tuning(method)
         ptr = None if out is None else out.ptr + offset_elems * np.dtype(self.cdtype).itemsize
         self._check(self.lib.hom_precession_block(self.handle, int(first), int(last),
                                                   self._flags(method), ptr, int(ld)))
@@ -393,7 +399,19 @@ class Plan:
         self._check(self.lib.hom_reduce_signal_noalloc(self.handle, states.ptr if states else None,
                                                        int(ld_states), int(n_adc), ptr, int(ld)))
 
+    def _apply_tuning(self, method):
+        """Mixed plans take the persisted spin tile of their shape class
+        (tuning.json) unless one was set explicitly."""
+        if self.precision != "mixed" or self._tile_explicit:
+            return
+        from . import tuning
+        tile = tuning.plan_spin_tile(self.n_coils, self._flags(method))
+        if tile and tile != self._tile_applied:
+            self._check(self.lib.hom_plan_set_spin_tile(self.handle, int(tile)))
+            self._tile_applied = tile
+
     def run(self, parts, method):
+        self._apply_tuning(method)
         pf = np.ascontiguousarray(parts[0], np.int64)
         pl = np.ascontiguousarray(parts[1], np.int64)
         rf = np.ascontiguousarray(parts[2], np.uint8)
@@ -428,6 +446,18 @@ class Plan:
         n = C.c_int64(0)
         self._check(self.lib.hom_plan_kernel_time(self.handle, KERNELS[name], C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    @property
+    def spin_tile(self) -> int:
+        """16-spin groups per spin tile of the mixed kernels."""
+        v = C.c_int(0)
+        self._check(self.lib.hom_plan_get_spin_tile(self.handle, C.byref(v)))
+        return v.value
+
+    @spin_tile.setter
+    def spin_tile(self, groups: int):
+        self._check(self.lib.hom_plan_set_spin_tile(self.handle, int(groups)))
+        self._tile_explicit = True
 
     def mixed_stats(self):
         """(segments, fp64_segments) of the last mixed run: precession segments

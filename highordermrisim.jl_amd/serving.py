@@ -99,11 +99,15 @@ class SimulationServer:
     ``step_budget`` bounds the spin-samples a request advances per step
     (smaller steps interleave requests more finely); ``gpt`` and
     ``main_grid`` are the fused kernel's spin-tile and grid shapes, which
-    :meth:`autotune` can choose."""
+    :meth:`autotune` can choose; left unset, they come from the persisted
+    tuning (tuning.json) when it has a server entry."""
 
     def __init__(self, device: int = 0, memory_bytes: int = 8 << 30, result_bytes: int = 1 << 30,
                  max_active: int = 64, step_budget: int | None = None, use_graphs: bool = True,
                  gpt: int | None = None, main_grid: int | None = None, threaded: bool = True):
+        if gpt is None and main_grid is None:
+            from . import tuning
+            gpt, main_grid = tuning.server_shape() or (None, None)
         self.srv = runtime.Server(device=device, device_bytes=memory_bytes, host_bytes=result_bytes,
                                   max_active=max_active, step_budget=step_budget or 0,
                                   use_graphs=use_graphs, gpt=gpt or 0, main_grid=main_grid or 0,

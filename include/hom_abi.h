@@ -202,6 +202,15 @@ int hom_buffer_download(hom_plan *plan, void *host, const void *dev, int64_t byt
 int hom_plan_set_timing(hom_plan *plan, int enable);
 int hom_plan_kernel_time(hom_plan *plan, int kernel_id, double *total_ms, int64_t *launches);
 
+/* Spin tile of the mixed kernels (no counterpart in the reference, whose CUDA
+ * launch shapes are fixed): 16-spin groups per workgroup work item, a power
+ * of two in [16, 4096] (default 256).  Mixed plans only; takes effect from the
+ * next hom_run / hom_precession_block.  The shipped tuner
+ * (highordermrisim.jl_amd/tune.py) picks it per coil-block count and
+ * field-order class and persists the choice. */
+int hom_plan_set_spin_tile(hom_plan *plan, int groups);
+int hom_plan_get_spin_tile(const hom_plan *plan, int *groups);
+
 /* Diagnostics of the last hom_run with precision "mixed" (no counterpart in
  * the reference): the number of free-precession segments that acquired
  * samples, and how many of them ran the all-FP64 phase body because their

@@ -39,9 +39,10 @@ def oracle_signal(obj, seq, first, count, coils, last_node, method):
     return sig
 
 
-def plan_signal(obj, seq, first, count, coils, last_node, method):
+def plan_signal(obj, seq, first, count, coils, last_node, method, spin_tile=256):
     """The same request through the single-plan mixed path, with the server's
-    partition (maximal RF / precession runs)."""
+    partition (maximal RF / precession runs) and spin tile (the FP32 partial
+    sums run over one spin tile)."""
     o = sub_phantom(obj, first, count, coils)
     v = seq.view(0, last_node)
     exc = np.asarray(v.excitation, bool)
@@ -56,6 +57,7 @@ def plan_signal(obj, seq, first, count, coils, last_node, method):
     parts = (np.array(pf), np.array(pl), np.array(rf, np.uint8))
     nadc = max([v.block_adc(x, y) for x, y in zip(pf, pl)] + [1])
     with Plan("mixed", len(o), coils, v.n_nodes, nadc) as plan:
+        plan.spin_tile = spin_tile
         plan.upload_phantom(o)
         plan.upload_sequence(v)
         plan.run(parts, method)
@@ -95,7 +97,8 @@ def test_serve_matches_oracle_and_plan(inputs):
             ref = oracle_signal(obj, seq, first, count, coils, last, m)
             e = nrmse(sig, ref)
             assert e <= NRMSE_BAR, f"{case}: NRMSE vs oracle {e:.3e}"
-            e2 = nrmse(sig, plan_signal(obj, seq, first, count, coils, last, m))
+            e2 = nrmse(sig, plan_signal(obj, seq, first, count, coils, last, m,
+                                        srv.config()["gpt"]))
             assert e2 <= 1e-12, f"{case}: NRMSE vs plan {e2:.3e}"
 
 
