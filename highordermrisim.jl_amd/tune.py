@@ -4,10 +4,11 @@
 
 For each shape class of the plan path (coil blocks 1..4 of a 32-coil pass x
 offset MFMAs 1..2, probed at 8/16/24/32 coils and field orders "1100" /
-"1111") it times ``k_mix_main`` over one RF + readout run of a probe phantom
-for every candidate spin tile (HIP events on the plan's stream, median of
-three runs after one warm-up) and keeps the fastest; for the server it times
-the persistent kernel over (spin tile, grid) pairs through
+"1111") it times ``k_mix_main`` plus ``k_mix_finalize`` (a smaller spin tile
+means more spin-tile partials to sum) over one RF + readout run of a probe
+phantom for every candidate spin tile (HIP events on the plan's stream,
+median of three runs after one warm-up) and keeps the fastest; for the
+server it times the persistent kernel over (spin tile, grid) pairs through
 ``hom_server_autotune``.  Run it on the GPU the library will serve; the file
 it writes is what :mod:`tuning` loads.
 """
@@ -50,15 +51,16 @@ def time_plan(obj, seq, parts, method, tile, runs=3):
             plan.run(parts, method)
             plan.synchronize()
             ms, n = plan.kernel_time("mixed_segment")
+            fms, fn = plan.kernel_time("mixed_finalize")
             plan.set_timing(False)
-            times.append(ms / max(n, 1))
+            times.append(ms / max(n, 1) + fms / max(fn, 1))
         return sorted(times)[len(times) // 2]
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=str(HERE / "tuning.json"))
-    ap.add_argument("--spins", type=int, default=1 << 18)
+    ap.add_argument("--spins", type=int, default=1 << 20)
     ap.add_argument("--samples", type=int, default=8000)
     ap.add_argument("--tiles", default="64,128,256,512")
     ap.add_argument("--no-server", action="store_true")
