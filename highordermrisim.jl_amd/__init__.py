@@ -11,7 +11,7 @@ Layers:
 - ``simulate``   the reference's ``simulate`` / ``prepare_simulation`` surface
                  on top of the ABI;
 - ``serving``    continuous-batching request loop over the ABI's server;
-- ``trajectory`` ``get_kspace`` (native ``hom_kspace``), ``grad2traj``, ``traj2grad``;
+- ``trajectory`` ``get_kspace`` (native ``hom_kspace``), ``rf_flip_angles``;
 - ``sequence``   block descriptions and ``discretize`` (native ``hom_discretize``).
 """
 from .types import (GAMMA, BlochHighOrder, DimensionMismatch, HO_DiscreteSequence,  # noqa: F401
@@ -28,8 +28,8 @@ __all__ += ["Mag", "PreparedSimulation", "default_sim_params", "prepare_simulati
             "setup_simulation", "simulate"]
 
 from . import serving  # noqa: E402,F401
-from .trajectory import get_kspace, grad2traj, rf_flip_angles, traj2grad  # noqa: E402,F401
+from .trajectory import get_kspace, rf_flip_angles  # noqa: E402,F401
 from .sequence import SeqBlock, discretize  # noqa: E402,F401
 
-__all__ += ["serving", "get_kspace", "grad2traj", "rf_flip_angles", "traj2grad", "SeqBlock",
+__all__ += ["serving", "get_kspace", "rf_flip_angles", "SeqBlock",
             "discretize"]

@@ -1,7 +1,6 @@
-"""Encoding trajectories (host code in libhom.so, no GPU): grad2traj /
-traj2grad as reference test/unit/test_trajectory_utils.jl:6-22, and
-get_kspace (reference src/datatypes/HO_Sequence.jl:221-287) against closed
-forms on discretized sequences: constant and ramp gradients integrate exactly
+"""Encoding trajectories (host code in libhom.so, no GPU): get_kspace
+(reference src/datatypes/HO_Sequence.jl:221-287) against closed forms on
+discretized sequences: constant and ramp gradients integrate exactly
 under the trapezoid rule, an excitation restarts the integral at the pulse,
 a refocusing pulse negates it, skip_rf continues through it, ADC rows are the
 values at the ADC nodes, shapes as test/unit/test_koma_compatibility.jl:46-53.
@@ -10,29 +9,9 @@ its tree): parity unpinned beyond these closed forms."""
 import numpy as np
 import pytest
 
-from hom_amd import get_kspace, grad2traj, rf_flip_angles, traj2grad
+from hom_amd import get_kspace, rf_flip_angles
 from hom_amd.types import GAMMA, DimensionMismatch, HO_DiscreteSequence
 from hom_amd.workloads import FreeBlock, RFBlock, build_sequence
-
-
-def test_trajectory_helpers():
-    dt = 0.25
-    values = np.arange(12.0).reshape(3, 4).T  # Julia reshape(0:11, 4, 3)
-    np.testing.assert_array_equal(traj2grad(values, dt), np.diff(values, axis=0) / dt)
-    expected = np.vstack([np.zeros((1, 3)),
-                          np.cumsum((values[:3] + values[1:]) * (dt / 2), axis=0)])
-    np.testing.assert_array_equal(grad2traj(values, dt), expected)
-    np.testing.assert_array_equal(grad2traj(values.T, dt, dim=1), expected.T)
-    assert grad2traj(np.zeros((0, 3)), dt).shape == (0, 3)
-    assert traj2grad(np.zeros((0, 3)), dt).shape == (0, 3)
-    assert traj2grad(np.ones((1, 3)), dt).shape == (0, 3)
-    for op in (grad2traj, traj2grad):
-        for bad in (0, -1, np.inf, np.nan):
-            with pytest.raises(ValueError):
-                op(values, bad)
-        for dim in (-1, 2):
-            with pytest.raises(ValueError):
-                op(values, dt, dim=dim)
 
 
 def grad_const(g):
