@@ -479,7 +479,7 @@ struct MainOps {
     const float *lm;
     const double *seed;
     const half8_t *bop;
-    const double *kref;   // per tile: 2 references x (x, y, z, dw) Kr, FP64
+    const double *kref;   // per tile: kRefsPerTile references x (x, y, z, dw) Kr, FP64
     const half8_t *bho;
     const float *tk;
     float *partials;
