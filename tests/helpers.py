@@ -29,6 +29,17 @@ def nrmse(a, ref):
     return float(np.linalg.norm((np.asarray(a) - ref).ravel()) / np.linalg.norm(ref.ravel()))
 
 
+def assert_signal(sig, ref, precision, what=""):
+    """Signal against the f64 oracle: the reference's isapprox (f32 3e-5, f64
+    1e-11, atol = rtol, norm-wise) and, for f64, NRMSE <= 1e-12 (the bar the
+    f64 drop-in is held to on every workload)."""
+    t = tol(precision)
+    assert_isapprox(sig, ref, t, t, what)
+    if precision == "f64" and np.linalg.norm(np.asarray(ref).ravel()) > 0:
+        e = nrmse(sig, ref)
+        assert e <= 1e-12, f"{what}: f64 NRMSE {e:.3e} above 1e-12"
+
+
 def round_f32(obj, seq):
     """Round every input to float32 (the reference converts inputs with f32()
     before the f32 path runs), keeping float64 containers for the oracle."""

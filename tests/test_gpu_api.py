@@ -11,7 +11,7 @@ from hom_amd.raw import RawAcquisitionData
 from hom_amd import BlochHighOrder, HO_Phantom, prepare_simulation, simulate
 from hom_amd.types import GAMMA, get_sim_ranges
 from hom_amd.workloads import FreeBlock, RFBlock, build_sequence
-from helpers import assert_isapprox, nrmse, print_case, require_gpu
+from helpers import assert_isapprox, assert_signal, nrmse, print_case, require_gpu
 
 pytestmark = pytest.mark.gpu
 TAU = 1e-4
@@ -124,8 +124,7 @@ def test_high_order_compatibility_all_coil_modes(precision):
             if precision == "mixed":
                 assert nrmse(sig[:, :, 0], ref) <= 1e-5
             else:
-                t = 3e-5 if precision == "f32" else 1e-11
-                assert_isapprox(sig[:, :, 0], ref, t, t)
+                assert_signal(sig[:, :, 0], ref, precision)
             assert np.all(np.isfinite(sig))
 
 

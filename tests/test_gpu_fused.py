@@ -14,7 +14,7 @@ import oracle
 from hom_amd.runtime import Plan
 from hom_amd.types import GAMMA, BlochHighOrder, DimensionMismatch, get_sim_ranges
 from hom_amd.workloads import high_order_workload, make_config
-from helpers import assert_isapprox, nrmse, print_case, require_gpu, round_f32, tol
+from helpers import assert_isapprox, assert_signal, nrmse, print_case, require_gpu, round_f32, tol
 from test_gpu_plain import make_plan, run_block
 
 pytestmark = pytest.mark.gpu
@@ -59,7 +59,7 @@ def test_fused_precession_parity(precision, mode):
                 sig = run_block(plan, "precession", 0, last, method, nadc, ncoils)
                 mxy, mz = plan.get_state()
             if nadc:
-                assert_isapprox(sig, ref, t, t, "signal")
+                assert_signal(sig, ref, precision, "signal")
             assert_isapprox(mxy, rmxy, t, t, "Mxy")
             assert_isapprox(mz, rmz, t, t, "Mz")
 
@@ -77,8 +77,7 @@ def test_fused_strided_output(precision):
             plan.set_state(mxy0, mz0)
             full = run_block(plan, "precession", 0, nsteps, m, nsteps, 3, ld=100, row0=16,
                              sentinel=complex(7, -3))
-        t = tol(precision)
-        assert_isapprox(full[16:16 + nsteps], ref, t, t, "rows")
+        assert_signal(full[16:16 + nsteps], ref, precision, "rows")
         assert np.all(full[:16] == complex(7, -3))
         assert np.all(full[16 + nsteps:] == complex(7, -3))
 
@@ -122,7 +121,7 @@ def test_fused_run_with_rf(precision):
         err = np.linalg.norm(sig - ref) / np.linalg.norm(ref)
         print(f"fused-run {precision} {mode} NRMSE {err:.3e}", flush=True)
         if precision == "f64":
-            assert_isapprox(sig, ref, 1e-11, 1e-11, mode)
+            assert_signal(sig, ref, "f64", mode)
             assert_isapprox(mxy, rmxy, 1e-11, 1e-11, "Mxy")
             assert_isapprox(mz, rmz, 1e-11, 1e-11, "Mz")
         else:
@@ -141,7 +140,7 @@ def test_fused_rf_adc_capacity_and_empty_phantom():
     with make_plan("f64", obj, seq, 8, max_rf_adc=8, coil_combination="fused_warp") as plan:
         plan.set_state(mxy0, mz0)
         sig = run_block(plan, "excitation", 0, 8, m, 8, 2)
-    assert_isapprox(sig, ref, 1e-11, 1e-11, "RF ADC")
+    assert_signal(sig, ref, "f64", "RF ADC")
     # no spins: the fused block writes zeros
     from hom_amd.types import HO_Phantom
     _, seq0, _, _ = high_order_workload(nspins=4, nsteps=20, ncoils=3)
@@ -166,7 +165,7 @@ def test_failed_allocation_leaves_no_stale_error():
                 plan.buffer(1 << 50)
             plan.set_state(mxy0, mz0)
             sig = run_block(plan, "precession", 0, 20, m, 20, 3)
-        assert_isapprox(sig, ref, 1e-11, 1e-11, mode)
+        assert_signal(sig, ref, "f64", mode)
 
 
 def test_mixed_plan_state_buffer_is_rf_only():

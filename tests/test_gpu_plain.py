@@ -15,7 +15,7 @@ import oracle
 from hom_amd.runtime import Plan
 from hom_amd.types import GAMMA, BlochHighOrder, get_sim_ranges
 from hom_amd.workloads import high_order_workload
-from helpers import assert_isapprox, print_case, require_gpu, round_f32, tol
+from helpers import assert_isapprox, assert_signal, print_case, require_gpu, round_f32, tol
 
 pytestmark = pytest.mark.gpu
 
@@ -65,7 +65,7 @@ def check_precession(precision, mask, ncoils=32, adc=True, dw_prec=True):
         mxy, mz = plan.get_state()
     t = tol(precision)
     if nadc:
-        assert_isapprox(sig, ref, t, t, "signal")
+        assert_signal(sig, ref, precision, "signal")
     assert_isapprox(mxy, rmxy, t, t, "Mxy")
     assert_isapprox(mz, rmz, t, t, "Mz")
 
@@ -126,7 +126,7 @@ def test_excitation(precision):
                 sig = run_block(plan, "excitation", 0, seq.n_nodes - 1, method, 32, ncoils)
                 mxy, mz = plan.get_state()
             t = tol(precision)
-            assert_isapprox(sig, ref, t, t, "signal")
+            assert_signal(sig, ref, precision, "signal")
             assert_isapprox(mxy, rmxy, t, t, "Mxy")
             assert_isapprox(mz, rmz, t, t, "Mz")
 
@@ -148,8 +148,7 @@ def test_strided_signal_views(precision):
             plan.set_state(mxy0, mz0)
             full = run_block(plan, "precession", 0, nsteps, m, nsteps, 3, ld=100, row0=16,
                              sentinel=complex(7, -3))
-        t = tol(precision)
-        assert_isapprox(full[16:16 + nsteps], ref, t, t, "rows")
+        assert_signal(full[16:16 + nsteps], ref, precision, "rows")
         assert np.all(full[:16] == complex(7, -3))
         assert np.all(full[16 + nsteps:] == complex(7, -3))
 
@@ -236,7 +235,7 @@ def test_block_length_invariance_and_run():
             plan.run(parts, m)
             sig = plan.download_signal()
             mxy, mz = plan.get_state()
-        assert_isapprox(sig, ref, 1e-11, 1e-11, f"L={L}")
+        assert_signal(sig, ref, "f64", f"L={L}")
         assert_isapprox(mxy, rmxy, 1e-11, 1e-11, "Mxy")
         assert_isapprox(mz, rmz, 1e-11, 1e-11, "Mz")
 
@@ -258,6 +257,6 @@ def test_rf_and_precession_run_small_config():
         err = np.linalg.norm(sig - ref) / np.linalg.norm(ref)
         print(f"config0-sub {precision} NRMSE {err:.3e}", flush=True)
         if precision == "f64":
-            assert_isapprox(sig, ref, 1e-11, 1e-11, precision)
+            assert_signal(sig, ref, "f64", precision)
         else:
             assert err < 1e-3

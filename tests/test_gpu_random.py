@@ -19,7 +19,7 @@ import oracle
 from hom_amd.runtime import Plan
 from hom_amd.types import GAMMA, BlochHighOrder, HO_Phantom, get_sim_ranges
 from hom_amd.workloads import FreeBlock, RFBlock, build_sequence
-from helpers import assert_isapprox, nrmse, print_case, require_gpu
+from helpers import assert_isapprox, assert_signal, nrmse, print_case, require_gpu
 
 pytestmark = pytest.mark.gpu
 
@@ -93,6 +93,6 @@ def test_random_case(seed):
     assert_isapprox(mxy, rmxy, 1e-11, 1e-11, "mixed Mxy")
     assert_isapprox(mz, rmz, 1e-11, 1e-11, "mixed Mz")
     sig, mxy, mz = run("f64", obj, seq, parts, method)
-    assert_isapprox(sig, ref, 1e-11, 1e-11, "f64 signal")
+    assert_signal(sig, ref, "f64", "f64 signal")
     assert_isapprox(mxy, rmxy, 1e-11, 1e-11, "f64 Mxy")
     assert_isapprox(mz, rmz, 1e-11, 1e-11, "f64 Mz")
