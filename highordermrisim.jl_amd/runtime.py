@@ -401,12 +401,13 @@ class Plan:
 
     def _apply_tuning(self, method):
         """Mixed plans take the persisted spin tile of their shape class
-        (tuning.json) unless one was set explicitly."""
+        (tuning.json), or the library default when the file has none, unless
+        one was set explicitly."""
         if self.precision != "mixed" or self._tile_explicit:
             return
         from . import tuning
-        tile = tuning.plan_spin_tile(self.n_coils, self._flags(method))
-        if tile and tile != self._tile_applied:
+        tile = tuning.plan_spin_tile(self.n_coils, self._flags(method)) or tuning.DEFAULT_SPIN_TILE
+        if tile != self._tile_applied:
             self._check(self.lib.hom_plan_set_spin_tile(self.handle, int(tile)))
             self._tile_applied = tile
 

@@ -27,6 +27,7 @@ PATH = Path(__file__).resolve().with_name("tuning.json")
 
 HOM_HO2, HOM_HO3, HOM_DW_PRECESSION = 1 << 2, 1 << 3, 1 << 5
 SPIN_TILES = (16, 32, 64, 128, 256, 512, 1024, 2048, 4096)
+DEFAULT_SPIN_TILE = 256  # the library's own default (hom_plan_create)
 
 _cache: dict = {}
 
