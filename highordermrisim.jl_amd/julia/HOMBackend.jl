@@ -86,10 +86,12 @@ f64vec(x) = Vector{Float64}(x)
 u8vec(x) = Vector{UInt8}(collect(x))
 
 """
-    HOMWorkspace(obj, hoseqd, max_adc; precision="f64", coil_combination="gemm",
-                 max_rf_adc=max_adc, groupsize=256, adc_tile=8, coil_tile=1, device=0, gamma=γ)
+    HOMWorkspace(obj, hoseqd, max_adc; gamma, precision="f64", coil_combination="gemm",
+                 max_rf_adc=max_adc, groupsize=256, adc_tile=8, coil_tile=1, device=0)
 
-Device-resident plan for one phantom and one sampled sequence: the argument
+Device-resident plan for one phantom and one sampled sequence (`gamma`, the
+gyromagnetic ratio in Hz/T, is required: the caller passes the package's γ):
+the argument
 checks of HighOrderGPUWorkspace (HighOrderGPU.jl:19-35, including the fused
 groupsize / tile / shared-memory checks through hom_check_workspace_options)
 before any device work, then the upload of the phantom, CSM and sequence
@@ -217,26 +219,32 @@ function hom_simulate_shard(obj, hoseqd, params, plan; callbacks=(), gamma, M)
                 ws.handle, sig, max(total, 1)), ws.handle)
         else
             sig = zeros(T, total, nc)
-            bytes = sizeof(T) * total * nc
+            # one block's rows (at most max_adc x Nc, column stride = its ADC
+            # count) go to a device buffer and are copied into sig's rows
             check(ccall((:hom_buffer_alloc, libhom), Cint, (Ptr{Cvoid}, Int64, Ptr{Ptr{Cvoid}}),
-                ws.handle, bytes, dev), ws.handle)
+                ws.handle, sizeof(T) * max(max_adc, 1) * nc, dev), ws.handle)
             row = 0
             rfs = 0
-            for (b, p) in enumerate(parts)
-                fn = excitation_bool[b] ? :hom_excitation_block : :hom_precession_block
-                out = adc_counts[b] == 0 ? C_NULL : dev[] + sizeof(T) * row
+            for b in eachindex(parts)
+                n = adc_counts[b]
+                out = n == 0 ? C_NULL : dev[]
                 st = excitation_bool[b] ?
                     ccall((:hom_excitation_block, libhom), Cint,
                         (Ptr{Cvoid}, Int64, Int64, Cint, Ptr{Cvoid}, Int64),
-                        ws.handle, pf[b], pl[b], flags, out, max(total, 1)) :
+                        ws.handle, pf[b], pl[b], flags, out, max(n, 1)) :
                     ccall((:hom_precession_block, libhom), Cint,
                         (Ptr{Cvoid}, Int64, Int64, Cint, Ptr{Cvoid}, Int64),
-                        ws.handle, pf[b], pl[b], flags, out, max(total, 1))
+                        ws.handle, pf[b], pl[b], flags, out, max(n, 1))
                 check(st, ws.handle)
-                row += adc_counts[b]
+                if n > 0
+                    rows = Matrix{T}(undef, n, nc)
+                    check(ccall((:hom_buffer_download, libhom), Cint,
+                        (Ptr{Cvoid}, Ptr{T}, Ptr{Cvoid}, Int64), ws.handle, rows, dev[],
+                        sizeof(T) * n * nc), ws.handle)
+                    sig[row+1:row+n, :] .= rows
+                end
+                row += n
                 rfs += excitation_bool[b] ? 1 : 0
-                check(ccall((:hom_buffer_download, libhom), Cint,
-                    (Ptr{Cvoid}, Ptr{T}, Ptr{Cvoid}, Int64), ws.handle, sig, dev[], bytes), ws.handle)
                 get_state!(M, ws)
                 progress = (; block=b, Nblocks=length(parts), rfs, samples=row + 1)
                 for cb in callbacks

@@ -138,38 +138,41 @@ def _run_shard(plan: Plan, setup: SimulationSetup, callbacks=(), initial=None):
         sig = plan.download_signal()
     else:
         # Per-block stepping with completed data for callbacks (SimulatorCore.jl:109-149).
+        # Each block writes its rows (column stride = its ADC count) into a
+        # device buffer of max_adc x Nc, copied into the host signal's rows.
         total = int(setup.adc_counts.sum())
         nc = setup.obj.n_coils
-        buf = plan.buffer(max(total * nc * np.dtype(plan.cdtype).itemsize, 16))
+        itemsize = np.dtype(plan.cdtype).itemsize
+        buf = plan.buffer(max(max(int(setup.max_adc), 1) * nc * itemsize, 16))
         sig = np.zeros((total, nc), dtype=plan.cdtype, order="F")
-        row = 0
-        rfs = 0
-        nparts = len(setup.parts[0])
-        for b in range(nparts):
-            first, last, rf = (int(setup.parts[0][b]), int(setup.parts[1][b]), bool(setup.parts[2][b]))
-            nadc = int(setup.adc_counts[b])
-            fn = plan.excitation_block if rf else plan.precession_block
-            fn(first, last, method, buf if nadc else None, max(total, 1), offset_elems=row)
-            rfs += rf
-            row += nadc
-            plan.synchronize()
-            if callbacks:
-                flat = np.zeros(total * nc, dtype=plan.cdtype)
-                buf.download(flat)
-                sig = flat.reshape((total, nc), order="F")
-                xy, z = plan.get_state()
-                state = Mag(xy.copy(), z.copy())
-                progress = {"block": b + 1, "Nblocks": nparts, "rfs": rfs, "samples": row + 1}
-                blocks = {"excitation_bool": setup.parts[2].astype(bool), "parts": setup.parts}
-                data = {"obj": setup.obj, "seqd": setup.seqd, "sig": sig, "Xt": state}
-                for cb in callbacks:
-                    cb(progress, blocks, data, params)
-                if not (np.array_equal(state.xy, xy) and np.array_equal(state.z, z)):
-                    plan.set_state(state.xy, state.z)
-        flat = np.zeros(total * nc, dtype=plan.cdtype)
-        buf.download(flat)
-        sig = flat.reshape((total, nc), order="F")
-        buf.free()
+        try:
+            row = 0
+            rfs = 0
+            nparts = len(setup.parts[0])
+            for b in range(nparts):
+                first, last, rf = (int(setup.parts[0][b]), int(setup.parts[1][b]), bool(setup.parts[2][b]))
+                nadc = int(setup.adc_counts[b])
+                fn = plan.excitation_block if rf else plan.precession_block
+                fn(first, last, method, buf if nadc else None, max(nadc, 1))
+                if nadc:
+                    rows = np.zeros(nadc * nc, dtype=plan.cdtype)
+                    buf.download(rows)
+                    sig[row:row + nadc] = rows.reshape((nadc, nc), order="F")
+                rfs += rf
+                row += nadc
+                plan.synchronize()
+                if callbacks:
+                    xy, z = plan.get_state()
+                    state = Mag(xy.copy(), z.copy())
+                    progress = {"block": b + 1, "Nblocks": nparts, "rfs": rfs, "samples": row + 1}
+                    blocks = {"excitation_bool": setup.parts[2].astype(bool), "parts": setup.parts}
+                    data = {"obj": setup.obj, "seqd": setup.seqd, "sig": sig, "Xt": state}
+                    for cb in callbacks:
+                        cb(progress, blocks, data, params)
+                    if not (np.array_equal(state.xy, xy) and np.array_equal(state.z, z)):
+                        plan.set_state(state.xy, state.z)
+        finally:
+            buf.free()
     seconds = time.perf_counter() - t0
     xy, z = plan.get_state()
     return sig, Mag(xy, z), seconds

@@ -115,12 +115,19 @@ def test_high_order_compatibility_all_coil_modes(precision):
         ref, _, _ = oracle.run(obj, seq, parts, m.flags, GAMMA)
         for mode in ("gemm", "reduction", "fused_warp", "fused_tiled"):
             print_case("compat", precision, mask, mode)
-            count = [0]
+            snaps = []
+
+            def seen(progress, blocks, data, sim_params):
+                # the rows of the blocks done so far, as the callback sees them
+                snaps.append(data["sig"][:progress["samples"] - 1].copy())
+
             sig = simulate(obj, seq, sim_params=params(precision=precision, sim_method=m,
                                                        coil_combination=mode, max_block_length=2,
                                                        max_rf_block_length=1),
-                           callbacks=[lambda *a: count.__setitem__(0, count[0] + 1)])
-            assert count[0] > 0
+                           callbacks=[seen])
+            assert len(snaps) > 0
+            for rows in snaps:
+                np.testing.assert_array_equal(rows, sig[:rows.shape[0], :, 0])
             if precision == "mixed":
                 assert nrmse(sig[:, :, 0], ref) <= 1e-5
             else:
