@@ -265,6 +265,11 @@ static int hom_plan_create_impl(hom_plan **out, int device, int precision, int64
         delete p;
         return set_error(nullptr, HOM_ERR_DEVICE, "hipStreamCreate failed.");
     }
+    if (hipDeviceGetAttribute(&p->cu_count, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        p->cu_count <= 0) {
+        (void)hipGetLastError();
+        p->cu_count = 256;  // MI355X
+    }
     // this device's address of the allocation-free reduction's scratch array
     if (scratch_setup(device) != hipSuccess) {
         (void)hipStreamDestroy(p->stream);
@@ -673,17 +678,28 @@ static int hom_plan_set_timing_impl(hom_plan *p, int enable) {
     return HOM_OK;
 }
 
-int hom_plan_set_spin_tile(hom_plan *p, int groups) {
+static int set_spin_tile(hom_plan *p, int groups, bool fixed) {
     if (!p) return HOM_ERR_ARGUMENT;
     if (groups < 16 || groups > 4096 || (groups & (groups - 1)) != 0)
         return set_error(p, HOM_ERR_ARGUMENT, "The spin tile must be a power of two between 16 and 4096 groups.");
     p->spin_tile = groups;
+    p->spin_tile_fixed = fixed;
     return HOM_OK;
 }
+
+int hom_plan_set_spin_tile(hom_plan *p, int groups) { return set_spin_tile(p, groups, true); }
+
+int hom_plan_set_spin_tile_auto(hom_plan *p, int groups) { return set_spin_tile(p, groups, false); }
 
 int hom_plan_get_spin_tile(const hom_plan *p, int *groups) {
     if (!p || !groups) return HOM_ERR_ARGUMENT;
     *groups = p->spin_tile;
+    return HOM_OK;
+}
+
+int hom_plan_last_spin_tile(const hom_plan *p, int *groups) {
+    if (!p || !groups) return HOM_ERR_ARGUMENT;
+    *groups = p->last_spin_tile;
     return HOM_OK;
 }
 

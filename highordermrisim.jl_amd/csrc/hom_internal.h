@@ -105,8 +105,12 @@ struct hom_plan {
     int64_t signal_rows = 0, signal_capacity_rows = 0;
 
     hom::MixedWorkspace *mixed = nullptr;
-    // 16-spin groups per spin tile of the mixed kernels (hom_plan_set_spin_tile)
+    // 16-spin groups per spin tile of the mixed kernels: fixed, or an upper
+    // bound of the per-segment choice (hom_plan_set_spin_tile[_auto])
     int spin_tile = 256;
+    bool spin_tile_fixed = false;
+    int last_spin_tile = 0;  // the longest segment's tile in the last mixed run
+    int cu_count = 0;        // compute units of the plan's device
     // Last mixed run: precession segments with samples, and how many of them
     // ran the all-FP64 phase body.
     int64_t mixed_segments = 0, mixed_fp64_segments = 0;

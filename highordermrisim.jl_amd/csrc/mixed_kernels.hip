@@ -613,6 +613,33 @@ void mixed_free(hom_plan *p) {
 }
 
 namespace {
+// Spin tile of one segment of nsb sample blocks: the fixed tile, or the
+// largest power of two up to the bound whose work items fill the device's
+// workgroup slots (two per CU) in whole waves best.  A smaller tile is taken
+// only when it fills the waves better by more than 5 % (each halving doubles
+// the partials the finalize sums).  Measured on MI355X (main + finalize):
+// 10,000 spins x 1 coil x 5000 samples 0.366 ms at 256 groups, 0.095 at the
+// 32 this picks; 350,000 spins x 8 coils x 4 x 4750: 4.19 ms at 256, 3.90 at 64.
+int segment_spin_tile(const hom_plan *p, int64_t ngroups, int64_t nsb) {
+    if (p->spin_tile_fixed) return p->spin_tile;
+    const int64_t slots = 2 * (int64_t)std::max(p->cu_count, 1);
+    auto fill = [&](int g) {
+        const int64_t items = (ngroups + g - 1) / g * nsb;
+        const int64_t waves = std::max<int64_t>((items + slots - 1) / slots, 1);
+        return (double)items / (double)(waves * slots);
+    };
+    int best = p->spin_tile;
+    double best_fill = fill(best);
+    for (int g = p->spin_tile / 2; g >= 16; g /= 2) {
+        const double f = fill(g);
+        if (f > best_fill + 0.05) {
+            best = g;
+            best_fill = f;
+        }
+    }
+    return best;
+}
+
 // The block loop of the mixed path over parts [pf, pl] (rf marks excitation
 // parts), writing sum(ADC) x Nc complex double rows into sig (column stride ld).
 int mixed_exec(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *rf, int64_t np,
@@ -706,16 +733,26 @@ int mixed_exec(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t 
         W.table_flags = mflags;
         W.table_parts = key;
     }
-    // Partial and seed buffers sized for the largest segment.
-    const int gpt = p->spin_tile;  // 16-spin groups per spin tile
-    const int64_t stile_spins = (int64_t)gpt * 16;
-    const int64_t nstiles = std::max<int64_t>((ns + stile_spins - 1) / stile_spins, 1);
-    int64_t need = 0, need_seed = 0;
-    for (const Segment &s : W.segs)
-        if (!s.rf) {
-            need = std::max<int64_t>(need, nstiles * s.tiles * 16 * 16 * NCB);
-            need_seed = std::max<int64_t>(need_seed, s.nab * ngroups * 16 + 16);
+    // Spin tiles (16-spin groups per work item) of the segments; partial and
+    // seed buffers sized for the largest segment.
+    auto stiles_of = [&](int g) {
+        const int64_t stile_spins = (int64_t)g * 16;
+        return std::max<int64_t>((ns + stile_spins - 1) / stile_spins, 1);
+    };
+    std::vector<int> seg_gpt(W.segs.size(), p->spin_tile);
+    int64_t need = 0, need_seed = 0, longest = -1;
+    p->last_spin_tile = 0;
+    for (size_t i = 0; i < W.segs.size(); ++i) {
+        const Segment &s = W.segs[i];
+        if (s.rf) continue;
+        seg_gpt[i] = segment_spin_tile(p, ngroups, s.nsb);
+        need = std::max<int64_t>(need, stiles_of(seg_gpt[i]) * s.tiles * 16 * 16 * NCB);
+        need_seed = std::max<int64_t>(need_seed, s.nab * ngroups * 16 + 16);
+        if (s.nadc > longest) {
+            longest = s.nadc;
+            p->last_spin_tile = seg_gpt[i];
         }
+    }
     if (need > W.partial_cap || need_seed > W.seed_cap) {
         if (const int e = check_hip(p, hipStreamSynchronize(p->stream), "hipStreamSynchronize")) return e;
         if (need > W.partial_cap) {
@@ -738,7 +775,8 @@ int mixed_exec(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t 
         }
     }
 
-    for (const Segment &s : W.segs) {
+    for (size_t si = 0; si < W.segs.size(); ++si) {
+        const Segment &s = W.segs[si];
         if (s.rf) {
             // RF blocks: f64 per-block excitation (+ reduction of RF ADC samples).
             if (s.nadc > p->max_adc)
@@ -783,6 +821,8 @@ int mixed_exec(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t 
             }
             rc = check_hip(p, hipGetLastError(), "mixed prep");
             if (rc) return rc;
+            const int gpt = seg_gpt[si];
+            const int64_t nstiles = stiles_of(gpt);
             const int64_t nwork = s.nsb * nstiles;
             for (int cb0 = 0; cb0 < NCB; cb0 += 4) {
                 const int ncb = std::min(4, NCB - cb0);

@@ -39,7 +39,8 @@ ABI_SYMBOLS = (
     "hom_buffer_download", "hom_plan_set_timing", "hom_plan_kernel_time", "hom_kspace",
     "hom_kspace_rf_events", "hom_discretize", "hom_discrete_copy", "hom_discrete_free",
     "hom_plan_mixed_stats", "hom_reduce_signal_noalloc", "hom_check_workspace_options",
-    "hom_plan_set_spin_tile", "hom_plan_get_spin_tile",
+    "hom_plan_set_spin_tile", "hom_plan_set_spin_tile_auto", "hom_plan_get_spin_tile",
+    "hom_plan_last_spin_tile",
 )
 
 # Every symbol declared in include/hom_serve.h.
@@ -150,7 +151,9 @@ def load_library():
         "hom_plan_kernel_time": (C.c_int, [P, C.c_int, _dp, _i64p]),
         "hom_plan_mixed_stats": (C.c_int, [P, _i64p, _i64p]),
         "hom_plan_set_spin_tile": (C.c_int, [P, C.c_int]),
+        "hom_plan_set_spin_tile_auto": (C.c_int, [P, C.c_int]),
         "hom_plan_get_spin_tile": (C.c_int, [P, C.POINTER(C.c_int)]),
+        "hom_plan_last_spin_tile": (C.c_int, [P, C.POINTER(C.c_int)]),
         "hom_kspace": (C.c_int, [C.POINTER(SequenceHost), C.c_double, _u8p, C.c_int64, _dp, _dp,
                                  _dp, _dp]),
         "hom_kspace_rf_events": (C.c_int, [C.POINTER(SequenceHost), C.c_double, _i64p, _dp,
@@ -401,14 +404,15 @@ class Plan:
 
     def _apply_tuning(self, method):
         """Mixed plans take the persisted spin tile of their shape class
-        (tuning.json), or the library default when the file has none, unless
-        one was set explicitly."""
+        (tuning.json), or the library default when the file has none, as the
+        upper bound of the library's per-segment choice, unless a tile was
+        fixed explicitly."""
         if self.precision != "mixed" or self._tile_explicit:
             return
         from . import tuning
         tile = tuning.plan_spin_tile(self.n_coils, self._flags(method)) or tuning.DEFAULT_SPIN_TILE
         if tile != self._tile_applied:
-            self._check(self.lib.hom_plan_set_spin_tile(self.handle, int(tile)))
+            self._check(self.lib.hom_plan_set_spin_tile_auto(self.handle, int(tile)))
             self._tile_applied = tile
 
     def run(self, parts, method):
@@ -457,8 +461,16 @@ class Plan:
 
     @spin_tile.setter
     def spin_tile(self, groups: int):
+        """Fixes the tile for every segment (no per-segment choice)."""
         self._check(self.lib.hom_plan_set_spin_tile(self.handle, int(groups)))
         self._tile_explicit = True
+
+    @property
+    def last_spin_tile(self) -> int:
+        """Spin tile the longest segment of the last mixed run used."""
+        v = C.c_int(0)
+        self._check(self.lib.hom_plan_last_spin_tile(self.handle, C.byref(v)))
+        return v.value
 
     def mixed_stats(self):
         """(segments, fp64_segments) of the last mixed run: precession segments

@@ -204,12 +204,23 @@ int hom_plan_kernel_time(hom_plan *plan, int kernel_id, double *total_ms, int64_
 
 /* Spin tile of the mixed kernels (no counterpart in the reference, whose CUDA
  * launch shapes are fixed): 16-spin groups per workgroup work item, a power
- * of two in [16, 4096] (default 256).  Mixed plans only; takes effect from the
- * next hom_run / hom_precession_block.  The shipped tuner
- * (highordermrisim.jl_amd/tune.py) picks it per coil-block count and
- * field-order class and persists the choice. */
+ * of two in [16, 4096].  Mixed plans only; takes effect from the next hom_run
+ * / hom_precession_block.
+ *   hom_plan_set_spin_tile       fixes the tile for every segment.
+ *   hom_plan_set_spin_tile_auto  makes `groups` an upper bound: per segment
+ *                                the library takes the largest tile whose
+ *                                work items fill the GPU's workgroup slots
+ *                                in whole waves (smaller tiles only for
+ *                                small phantoms or short segments).  This is
+ *                                the default, with bound 256.
+ * hom_plan_get_spin_tile reads the set value; hom_plan_last_spin_tile the
+ * tile the longest segment of the last mixed run used.  The shipped tuner
+ * (highordermrisim.jl_amd/tune.py) measures the bound per coil-block count
+ * and field-order class and persists it. */
 int hom_plan_set_spin_tile(hom_plan *plan, int groups);
+int hom_plan_set_spin_tile_auto(hom_plan *plan, int groups);
 int hom_plan_get_spin_tile(const hom_plan *plan, int *groups);
+int hom_plan_last_spin_tile(const hom_plan *plan, int *groups);
 
 /* Diagnostics of the last hom_run with precision "mixed" (no counterpart in
  * the reference): the number of free-precession segments that acquired
