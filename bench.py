@@ -162,8 +162,10 @@ def main():
     from hom_amd import tuning
     ncb, nmh = tuning.shape_class(nc, method.flags)
     tuned_plan = {"source": "highordermrisim.jl_amd/tuning.json" if tuning.load() else None,
-                  "shape_class": {"ncb": ncb, "nmh": nmh}, "spin_tile": plan.spin_tile}
-    log(f"tuning: shape class ncb={ncb} nmh={nmh}, spin tile {plan.spin_tile} groups "
+                  "shape_class": {"ncb": ncb, "nmh": nmh}, "spin_tile_bound": plan.spin_tile,
+                  "spin_tile": plan.last_spin_tile}
+    log(f"tuning: shape class ncb={ncb} nmh={nmh}, spin tile {plan.last_spin_tile} groups "
+        f"(bound {plan.spin_tile}) "
         f"({tuned_plan['source'] or 'library default'})")
     value = spin_samples * args.steps / elapsed
     log(f"mixed: {value:.4e} spin-samples/s, {elapsed / args.steps * 1e3:.2f} ms/step; "
