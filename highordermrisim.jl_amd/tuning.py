@@ -8,7 +8,10 @@ GPU and read by :class:`runtime.Plan` (before every mixed run) and by
   ``ncb`` = ceil(2 Nc / 16) of one 32-coil pass (1..4) and offset MFMAs
   ``nmh`` per tile (1 or 2, fixed by the field orders and the off-resonance
   flag) -- with the spin tile (16-spin groups per workgroup item) whose
-  measured kernel time was lowest, and the times of every candidate;
+  measured kernel time was lowest on a 1M-spin probe, and the times of every
+  candidate.  A plan uses it as the upper bound of the library's per-segment
+  choice, which takes smaller tiles when the work items would not fill the
+  GPU;
 * ``server``: the persistent kernel's spin tile and grid.
 
 The reference has no counterpart: its CUDA launch shapes are fixed
