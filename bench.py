@@ -290,13 +290,17 @@ def main():
         with Plan("mixed", len(sub), nc, seq.n_nodes, max_adc) as ps:
             ps.upload_phantom(sub)
             ps.upload_sequence(seq)
+            # the timed run's spin tile, so each FP32 partial sum covers as
+            # many spins as there (the subsample alone would get a smaller one)
+            ps.spin_tile = tuned_plan["spin_tile"]
             ps.run(parts, method)
             sig = ps.download_signal()
         n = cfg.n_adc
         per = [float(np.linalg.norm(sig[i * n:(i + 1) * n] - ref[i * n:(i + 1) * n]) /
                      np.linalg.norm(ref[i * n:(i + 1) * n])) for i in range(cfg.n_interleaves)]
         accuracy = {"nrmse_vs_f64_oracle": float(np.linalg.norm(sig - ref) / np.linalg.norm(ref)),
-                    "nrmse_max_per_interleave": max(per), "spins_checked": len(sub)}
+                    "nrmse_max_per_interleave": max(per), "spins_checked": len(sub),
+                    "spin_tile": tuned_plan["spin_tile"]}
         # the plain drop-ins on the same spins
         for prec in ("f64", "f32"):
             with Plan(prec, len(sub), nc, seq.n_nodes, max_adc) as pd:

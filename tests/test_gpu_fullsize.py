@@ -10,7 +10,9 @@ any size:
 * additivity over spins: the signal of a phantom equals the sum of the
   signals of two halves split at a 4096-spin tile boundary (NRMSE <= 1e-6:
   the halves get their own power-of-two scales);
-* parity on a spin subsample against the f64 oracle (NRMSE <= 1e-5);
+* parity on a spin subsample against the f64 oracle (NRMSE <= 1e-5), run
+  with the full-size run's spin tile so each FP32 partial sum covers as many
+  spins as there;
 * the public simulate() with its default whole-run blocks gives the same
   signal as 32-node blocks (the mixed path fuses free precession anyway).
 """
@@ -26,18 +28,22 @@ from helpers import nrmse, print_case, require_gpu
 pytestmark = pytest.mark.gpu
 
 
-def run(obj, seq, parts, method, twice=False):
+def run(obj, seq, parts, method, twice=False, tile=None):
+    """(signal, signal of a second run or None, spin tile the run used)."""
     nadc = max(seq.block_adc(a, b) for a, b in zip(parts[0], parts[1]))
     with Plan("mixed", len(obj), obj.n_coils, seq.n_nodes, nadc) as plan:
         plan.upload_phantom(obj)
         plan.upload_sequence(seq)
+        if tile is not None:
+            plan.spin_tile = tile
         plan.run(parts, method)
         first = plan.download_signal()
+        used = plan.last_spin_tile
         if not twice:
-            return first, None
+            return first, None, used
         plan.reset_state()
         plan.run(parts, method)
-        return first, plan.download_signal()
+        return first, plan.download_signal(), used
 
 
 def scaled(obj, k):
@@ -52,7 +58,7 @@ def test_fullsize_properties(config, subsample):
     obj, seq, cfg = make_config(config)
     method = BlochHighOrder("1111")
     parts = get_sim_ranges(seq, 32, 32)
-    sig, again = run(obj, seq, parts, method, twice=True)
+    sig, again, tile = run(obj, seq, parts, method, twice=True)
     np.testing.assert_array_equal(sig, again)
     assert np.isfinite(sig).all() and np.abs(sig).max() > 0
 
@@ -65,18 +71,18 @@ def test_fullsize_properties(config, subsample):
     e = nrmse(mat[:, :, 0], sig)
     assert e <= 1e-12, f"simulate() with whole-readout blocks: NRMSE {e:.3e}"
 
-    sig2, _ = run(scaled(obj, 2.0), seq, parts, method)
+    sig2, _, _ = run(scaled(obj, 2.0), seq, parts, method)
     np.testing.assert_array_equal(sig2, 2.0 * sig)
 
     cut = (len(obj) // 2) // 4096 * 4096
-    a, _ = run(obj[:cut], seq, parts, method)
-    b, _ = run(obj[cut:], seq, parts, method)
+    a, _, _ = run(obj[:cut], seq, parts, method)
+    b, _, _ = run(obj[cut:], seq, parts, method)
     e = nrmse(a + b, sig)
     assert e <= 1e-6, f"additivity NRMSE {e:.3e}"
 
     sub = obj[::subsample]
     ref, _, _ = oracle.run(sub, seq, parts, method.flags, GAMMA, n_threads=16)
-    s_sub, _ = run(sub, seq, parts, method)
+    s_sub, _, _ = run(sub, seq, parts, method, tile=tile)
     n = cfg.n_adc
     worst = max(nrmse(s_sub[i * n:(i + 1) * n], ref[i * n:(i + 1) * n])
                 for i in range(cfg.n_interleaves))
