@@ -616,8 +616,8 @@ namespace {
 // Spin tile of one segment of nsb sample blocks: the fixed tile, or the
 // largest power of two up to the bound whose work items fill the device's
 // workgroup slots (two per CU) in whole waves best.  A smaller tile is taken
-// only when it fills the waves better by more than 5 % (each halving doubles
-// the partials the finalize sums).  Measured on MI355X (main + finalize):
+// only when it fills the waves more than 5 % better, relative (each halving
+// doubles the partials the finalize sums).  Measured on MI355X (main + finalize):
 // 10,000 spins x 1 coil x 5000 samples 0.366 ms at 256 groups, 0.095 at the
 // 32 this picks; 350,000 spins x 8 coils x 4 x 4750: 4.19 ms at 256, 3.90 at 64.
 int segment_spin_tile(const hom_plan *p, int64_t ngroups, int64_t nsb) {
@@ -632,7 +632,7 @@ int segment_spin_tile(const hom_plan *p, int64_t ngroups, int64_t nsb) {
     double best_fill = fill(best);
     for (int g = p->spin_tile / 2; g >= 16; g /= 2) {
         const double f = fill(g);
-        if (f > best_fill + 0.05) {
+        if (f > best_fill * 1.05) {
             best = g;
             best_fill = f;
         }
