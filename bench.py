@@ -75,6 +75,34 @@ def measured_traffic(spin_samples_per_launch: float, workload: str):
     return float(vals["bytes_per_spin_sample"]) * spin_samples_per_launch
 
 
+COUNTERS_PROFILE = ROOT / "profiles" / "k_mix_main-counters-3d-1M-32coil-16x8000.md"
+
+
+def measured_counters(spin_samples_per_launch: float, kernel_s: float, workload: str):
+    """rocprofv3 instruction counts of the fused kernel (committed counters
+    profile of this workload, per 16 x 16 tile) turned into rates with this
+    run's kernel time: FP16 matrix work, transcendentals and VALU issue against
+    MI355X's published peaks; None when the profile is absent or was taken on
+    another workload."""
+    try:
+        text = COUNTERS_PROFILE.read_text()
+    except OSError:
+        return None
+    vals = dict(re.findall(r"^- (\w+): ([0-9.eE+-]+|[\w.-]+)$", text, flags=re.M))
+    if vals.get("workload") != workload or kernel_s <= 0:
+        return None
+    tiles = spin_samples_per_launch / 256.0
+    f16 = float(vals["mfma_f16_flops_per_tile"]) * tiles / kernel_s
+    trans = float(vals["trans_insts_per_tile"]) * 64 * tiles / kernel_s
+    return {"source": str(COUNTERS_PROFILE.relative_to(ROOT)),
+            "mfma_f16_tflops": f16 / 1e12, "mfma_f16_frac_of_peak": f16 / 2.5e15,
+            "mfma_busy_frac_of_simd_cycles": float(vals["mfma_busy_frac"]),
+            "transcendentals_per_s": trans, "transcendental_frac_of_peak": trans / 9.8e12,
+            "valu_insts_per_tile": float(vals["valu_insts_per_tile"]),
+            "simd_cycles_per_tile": float(vals["simd_cycles_per_tile"]),
+            "shader_clock_ghz_under_load": float(vals["shader_clock_ghz"])}
+
+
 def ceiling(nc, nt):
     """Ceiling model: t_min per spin-sample = max(3*8*Nc / P_fp16_mfma,
     (2*Nt + 20) / P_fp32 + 3 / P_trans); algorithmic flops F = 8 Nc + 2 Nt + 14."""
@@ -197,7 +225,13 @@ def main():
             "formula": "t_min=max(24*Nc/P_fp16mfma,(2*Nt+20)/P_fp32+3/P_trans); "
                        "F_alg=8*Nc+2*Nt+14 per spin-sample",
         },
+        "counters": measured_counters(ss_per_launch, seg_ms / max(seg_n, 1) / 1e3, cfg.name),
     }
+    if roofline["counters"]:
+        c = roofline["counters"]
+        log(f"counters: FP16 matrix work {c['mfma_f16_tflops']:.0f} TFLOP/s "
+            f"({c['mfma_f16_frac_of_peak']:.2f} of peak), transcendentals "
+            f"{c['transcendentals_per_s']:.2e}/s ({c['transcendental_frac_of_peak']:.2f} of peak)")
     extra = {"finalize_ms_avg": fin_ms / max(fin_n, 1), "prep_ms_avg": prep_ms / max(prep_n, 1),
              "p50_step_ms": float(np.percentile(step_ms, 50)),
              "p99_step_ms": float(np.percentile(step_ms, 99))}
