@@ -118,8 +118,15 @@ def setup_simulation(obj: HO_Phantom, seqd: HO_DiscreteSequence, sim_params=None
     rf_counts = adc_counts[parts[2].astype(bool)]
     max_rf_adc = int(rf_counts.max()) if rf_counts.size else 0
     t_sim_parts = np.append(seqd.t[parts[0]], seqd.t[-1])
-    return SimulationSetup(obj, seqd, params, parts, adc_counts, max_adc, max_rf_adc,
-                           t_sim_parts, device)
+    setup = SimulationSetup(obj, seqd, params, parts, adc_counts, max_adc, max_rf_adc,
+                            t_sim_parts, device)
+    # what simulate_shard records before the block loop, so callbacks see it
+    # (Simulate.jl:168-175); params is this call's own copy
+    params["implementation"] = "kernel"
+    params["max_adc_per_block"] = max_adc
+    params["max_rf_adc_per_block"] = max_rf_adc
+    params["adc_state_buffer_samples"] = _state_buffer_samples(setup)
+    return setup
 
 
 def _run_shard(plan: Plan, setup: SimulationSetup, callbacks=(), initial=None):
@@ -246,6 +253,7 @@ class PreparedSimulation:
 
     def __init__(self, setup: SimulationSetup):
         self.setup = setup
+        setup.params["prepared"] = True  # PreparedSimulation.jl:104
         self._plan = _make_plan(setup)
         self._lock = threading.Lock()
         self._running = threading.local()
