@@ -22,10 +22,10 @@ __all__ = ["GAMMA", "BlochHighOrder", "DimensionMismatch", "HO_DiscreteSequence"
            "get_sim_ranges", "workloads"]
 
 from .simulate import (Mag, PreparedSimulation, default_sim_params, prepare_simulation,  # noqa: E402,F401
-                       setup_simulation, simulate)
+                       setup_simulation, simulate, simulation_spin_ranges)
 
 __all__ += ["Mag", "PreparedSimulation", "default_sim_params", "prepare_simulation",
-            "setup_simulation", "simulate"]
+            "setup_simulation", "simulate", "simulation_spin_ranges"]
 
 from . import serving  # noqa: E402,F401
 from .trajectory import get_kspace, rf_flip_angles  # noqa: E402,F401

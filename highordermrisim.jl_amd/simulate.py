@@ -58,6 +58,42 @@ class SimulationSetup:
     device: int
 
 
+def simulation_spin_ranges(nspins: int, nshards: int) -> list[range]:
+    """Contiguous spin shards [floor((i-1) N / k), floor(i N / k)) for i = 1..k,
+    0-based (src/simulation/DistributedSimulation.jl:28-31): how a multi-GPU
+    run would split the phantom."""
+    if not (isinstance(nspins, (int, np.integer)) and isinstance(nshards, (int, np.integer))
+            and 1 <= nshards <= nspins):
+        raise ValueError("Require 1 <= nshards <= nspins.")
+    return [range((i - 1) * nspins // nshards, i * nspins // nshards) for i in range(1, nshards + 1)]
+
+
+def select_device(params: dict, gpus, nspins: int) -> int:
+    """The device checks of set_simulation_devices!
+    (src/simulation/DistributedSimulation.jl:2-26) for this one-GPU build: gpus
+    is None or a list/tuple of distinct nonnegative integer IDs (not bools), at
+    most one per spin, each a visible device; more than one ID is refused (one
+    plan runs on one GPU).  Records gpu, gpu_device and gpus in params."""
+    if gpus is not None and not isinstance(gpus, (list, tuple)):
+        raise ValueError("gpus must be a vector or tuple of visible-device IDs.")
+    ids = list(gpus) if gpus is not None else [params.get("gpu_device") or 0]
+    if not ids or not all(isinstance(i, (int, np.integer)) and not isinstance(i, (bool, np.bool_))
+                          and i >= 0 for i in ids):
+        raise ValueError("gpus must contain nonnegative integer visible-device IDs.")
+    if len(set(ids)) != len(ids):
+        raise ValueError("gpus contains duplicate IDs.")
+    if len(ids) > nspins:
+        raise ValueError("Each GPU needs at least one spin.")
+    if len(ids) > 1:
+        raise ValueError("This build runs one simulation on one GPU; pass gpus=[id].")
+    if gpus is not None and ids[0] >= device_count():
+        raise ValueError("gpus contains an unavailable device ID.")
+    params["gpu"] = True
+    params["gpu_device"] = int(ids[0])
+    params["gpus"] = [int(ids[0])]
+    return int(ids[0])
+
+
 def setup_simulation(obj: HO_Phantom, seqd: HO_DiscreteSequence, sim_params=None, gpus=None):
     """Validation and block plan, in the reference's order
     (src/simulation/Simulate.jl:70-135); errors are raised before any device work."""
@@ -90,13 +126,7 @@ def setup_simulation(obj: HO_Phantom, seqd: HO_DiscreteSequence, sim_params=None
     if obj.n_coils < 1:
         raise ValueError("The CSM must contain at least one coil.")
     seqd.validate()
-    if gpus is not None:
-        gpus = list(gpus)
-        if len(gpus) != 1:
-            raise ValueError("This build runs one simulation on one GPU; pass gpus=[id].")
-        device = int(gpus[0])
-    else:
-        device = int(params.get("gpu_device") or 0)
+    device = select_device(params, gpus, len(obj))
     if "Nblocks" in sim_params:
         nb = sim_params["Nblocks"]
         if not (isinstance(nb, (int, np.integer)) and nb > 0):

@@ -51,3 +51,35 @@ def test_block_plan_and_legacy_nblocks():
     params = {"precision": "f64"}
     setup_simulation(obj, seq, params)
     assert params == {"precision": "f64"}
+
+
+def test_simulation_spin_ranges():
+    # test/unit/test_execution_options.jl:4-7 (1-based 1:2, 3:5, 6:8, 9:11)
+    ranges = hom_amd.simulation_spin_ranges(11, 4)
+    assert [(r.start, r.stop) for r in ranges] == [(0, 2), (2, 5), (5, 8), (8, 11)]
+    assert [i for r in ranges for i in r] == list(range(11))
+    with pytest.raises(ValueError):
+        hom_amd.simulation_spin_ranges(2, 3)
+    with pytest.raises(ValueError):
+        hom_amd.simulation_spin_ranges(5, 0)
+
+
+@pytest.mark.parametrize("gpus", [[], [0, 0], [-1], [True], [0.5], 0, "0", [0, 1, 2]])
+def test_rejects_bad_gpu_lists(gpus):
+    # test/unit/test_execution_options.jl:19-23: checked before any device work
+    obj, seq = compat()
+    with pytest.raises(ValueError):
+        setup_simulation(obj, seq, {"sim_method": BlochHighOrder("0000")}, gpus=gpus)
+
+
+def test_more_gpus_than_spins_and_string_keys():
+    obj, seq = compat()
+    with pytest.raises(ValueError, match="at least one spin"):
+        setup_simulation(obj, seq, {}, gpus=[0, 1, 2])
+    with pytest.raises(ValueError, match="string keys"):
+        setup_simulation(obj, seq, {"precision": "f64", 3: 1})
+    s = setup_simulation(obj, seq, {"sim_method": BlochHighOrder("0000")})
+    assert s.params["gpu"] is True and s.params["gpu_device"] == 0 and s.params["gpus"] == [0]
+    assert s.params["implementation"] == "kernel"
+    assert s.params["max_adc_per_block"] == s.max_adc
+    assert s.params["adc_state_buffer_samples"] == s.max_adc  # f32 "gemm": every block's ADC
