@@ -1,0 +1,97 @@
+"""The C ABI alone, from a C host.  tests/c_abi/hom_c_run (built by
+__graft_entry__.build() with gcc against include/hom_abi.h and libhom.so)
+creates a plan, uploads the phantom, CSM and sequence, runs the whole block
+loop with hom_run and downloads the signal, in f64, f32 and mixed precision,
+on configuration 1's workload at every 500th spin (RF + 4 x 4750 samples, 8
+coils).  Its signal equals the Python ctypes path's bit for bit (same
+library, same fixed spin tile), and the f64 one matches the f64 oracle at
+the reference's tolerance."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from hom_amd.runtime import Plan
+from hom_amd.types import GAMMA, BlochHighOrder, get_sim_ranges
+from hom_amd.workloads import make_config
+from helpers import assert_signal, print_case, require_gpu
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = Path(__file__).resolve().parent / "c_abi" / "hom_c_run"
+CODES = {"f32": 0, "f64": 1, "mixed": 2}
+TILE = 64
+
+
+def write_problem(path, precision, flags, obj, seq, parts, max_adc):
+    nn = seq.n_nodes
+    with open(path, "wb") as f:
+        np.array([CODES[precision], 0, flags, len(obj), obj.n_coils, nn, max_adc, 0, len(parts[0])],
+                 np.int64).tofile(f)
+        np.array([GAMMA], np.float64).tofile(f)
+        for a in (obj.x, obj.y, obj.z, obj.rho, obj.T1, obj.T2, obj.dw):
+            np.asarray(a, np.float64).tofile(f)
+        np.asfortranarray(obj.csm, dtype=np.complex128).ravel(order="F").view(np.float64).tofile(f)
+        for a in (seq.t, seq.dt, seq.gx, seq.gy, seq.gz):
+            np.asarray(a, np.float64).tofile(f)
+        np.asarray(seq.b1, np.complex128).view(np.float64).tofile(f)
+        for a in (seq.df, seq.psi):
+            np.asarray(a, np.float64).tofile(f)
+        np.asarray(seq.adc, np.uint8).tofile(f)
+        np.asarray(seq.excitation, np.uint8).tofile(f)
+        np.ascontiguousarray(seq.h, dtype=np.float64).tofile(f)   # 16 x nn, term-major
+        np.asarray(parts[0], np.int64).tofile(f)
+        np.asarray(parts[1], np.int64).tofile(f)
+        np.asarray(parts[2], np.uint8).tofile(f)
+
+
+def read_signal(path):
+    raw = np.fromfile(path, dtype=np.uint8)
+    rows, cols = raw[:16].view(np.int64)
+    return raw[16:].view(np.complex128).reshape((rows, cols), order="F")
+
+
+def test_c_host_matches_ctypes_and_oracle(tmp_path):
+    require_gpu()
+    assert DRIVER.exists(), "tests/c_abi/hom_c_run is missing: run __graft_entry__.build()"
+    obj, seq, _ = make_config(1, spin_subsample=500)
+    method = BlochHighOrder("1111")
+    parts = get_sim_ranges(seq, 32, 32)
+    max_adc = max(seq.block_adc(a, b) for a, b in zip(parts[0], parts[1]))
+    ref, _, _ = oracle.run(obj, seq, parts, method.flags, GAMMA)
+    for precision in ("f64", "f32", "mixed"):
+        print_case("c-abi", precision)
+        src, dst = tmp_path / f"in_{precision}.bin", tmp_path / f"out_{precision}.bin"
+        write_problem(src, precision, method.flags, obj, seq, parts, max_adc)
+        run = subprocess.run([str(DRIVER), str(src), str(dst), str(TILE)], capture_output=True,
+                             text=True, timeout=120)
+        assert run.returncode == 0, f"hom_c_run {precision}: {run.returncode} {run.stderr}"
+        c_sig = read_signal(dst)
+        with Plan(precision, len(obj), obj.n_coils, seq.n_nodes, max_adc, max_rf_adc=0) as plan:
+            plan.upload_phantom(obj)
+            plan.upload_sequence(seq)
+            plan.spin_tile = TILE
+            plan.run(parts, method)
+            py_sig = plan.download_signal().astype(np.complex128)
+        assert c_sig.shape == py_sig.shape == ref.shape
+        np.testing.assert_array_equal(c_sig, py_sig)
+        if precision == "f64":
+            assert_signal(c_sig, ref, "f64", "C host f64")
+
+
+def test_c_host_reports_abi_errors(tmp_path):
+    """A problem the ABI rejects (max_adc below a block's ADC count) makes the
+    C host exit with HOM_ERR_DIMENSION and the library's message."""
+    require_gpu()
+    assert DRIVER.exists(), "tests/c_abi/hom_c_run is missing: run __graft_entry__.build()"
+    obj, seq, _ = make_config(1, spin_subsample=500)
+    method = BlochHighOrder("1111")
+    parts = get_sim_ranges(seq, 32, 32)
+    src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
+    write_problem(src, "f64", method.flags, obj, seq, parts, 1)
+    run = subprocess.run([str(DRIVER), str(src), str(dst)], capture_output=True, text=True,
+                         timeout=120)
+    assert run.returncode == 2, (run.returncode, run.stderr)
+    assert "hom_run" in run.stderr and "max_adc" in run.stderr
