@@ -96,3 +96,29 @@ def test_random_case(seed):
     assert_signal(sig, ref, "f64", "f64 signal")
     assert_isapprox(mxy, rmxy, 1e-11, 1e-11, "f64 Mxy")
     assert_isapprox(mz, rmz, 1e-11, 1e-11, "f64 Mz")
+
+
+LARGE_SEEDS = [3001, 3003, 3016, 3020]
+
+
+@pytest.mark.parametrize("seed", LARGE_SEEDS)
+def test_random_case_large_phantom(seed):
+    """The random sequences of test_random_case on 20,000-120,000 random spins
+    (many spin tiles, automatic tiles above the smallest), mixed path against
+    the f64 oracle."""
+    require_gpu()
+    obj, seq, method, parts, L = random_case(seed)
+    rng = np.random.default_rng(seed + 99)
+    ns, nc = int(rng.integers(20_000, 120_000)), obj.n_coils
+    big = HO_Phantom(x=rng.uniform(-0.1, 0.1, ns), y=rng.uniform(-0.1, 0.1, ns),
+                     z=rng.uniform(-0.035, 0.035, ns), rho=rng.uniform(0.0, 1.0, ns),
+                     T1=rng.uniform(0.5, 2.0, ns), T2=rng.uniform(0.02, 0.2, ns),
+                     dw=2 * np.pi * rng.normal(0.0, 20.0, ns),
+                     csm=np.asfortranarray(0.5 * (rng.normal(size=(ns, nc)) + 1j * rng.normal(size=(ns, nc)))))
+    print_case("random-large", seed, ns, nc, int(seq.adc.sum()), method.name, L)
+    ref, rmxy, rmz = oracle.run(big, seq, parts, method.flags, GAMMA, n_threads=16)
+    sig, mxy, mz = run("mixed", big, seq, parts, method)
+    assert sig.shape == ref.shape
+    assert nrmse(sig, ref) <= 1e-5, f"mixed NRMSE {nrmse(sig, ref):.3e}"
+    assert_isapprox(mxy, rmxy, 1e-11, 1e-11, "mixed Mxy")
+    assert_isapprox(mz, rmz, 1e-11, 1e-11, "mixed Mz")
