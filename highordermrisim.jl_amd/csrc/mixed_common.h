@@ -601,8 +601,9 @@ __device__ inline void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // v
 // wave (rotating) turns them into group g + 1's first-order anchors for the
 // 16 tile references of the sample block (one FP64 MFMA) and writes them
 // to the other anchor buffer, which the barrier publishes.  The per-tile
-// offset B operands stay in registers for the whole work item: LDS
-// bandwidth, not LDS space, is what the group loop runs short of.  Every wave
+// offset B operands stay in registers for the whole work item (reading
+// them from LDS every group would nearly double the group loop's LDS
+// reads; fewer LDS bytes per spin-sample measured faster).  Every wave
 // takes part in the copies and barriers, with or without tiles; the body ends
 // on a barrier, so a persistent caller may reuse the LDS at once.
 template <int NMH, int NCB>
