@@ -69,8 +69,8 @@ constexpr float kHOBMax = 512.0f;
 // First-order offsets: A = b_j * 2^a_j with max |A| in (2^13, 2^14], B =
 // dKr_j * 2^(kHOOut - a_j) (the same product scale, one accumulator), split
 // three ways; |B| <= 2^8 bounds each term by 4 revolutions, which FP32 holds
-// to 2.4e-7 revolutions (configuration 2 reaches about 2 revolutions at
-// 30 mT/m, 0.1 m and 8 samples of 2 us).
+// to 2.4e-7 revolutions (a 0.2 m field of view reaches about 2 revolutions
+// at 30 mT/m, 0.1 m and 8 samples of 2 us).
 constexpr int kFOA = 14;
 constexpr float kFOBMax = 256.0f;
 // First-order references per 16-sample tile (sample 8).

@@ -267,7 +267,10 @@ def config_sequence(cfg: Config, rng, gamma=GAMMA):
 
 
 def make_config(index: int, spin_subsample: int | None = None, gamma=GAMMA):
-    """Builds (phantom, sequence) for configuration 0, 1, 2 or 4.  With
+    """Builds (phantom, sequence, config) for workload 0 (2D, 10,000 spins, 1
+    coil, one 5000-sample spiral), 1 (350,000 spins, 8 coils, 4 x 4750), 2
+    (1,048,576 spins, 32 coils, 16 x 8000) or 4 (4,000,000 spins, 32 coils,
+    32 x 10000; RF every TR).  With
     ``spin_subsample`` the phantom keeps every k-th spin of the full grid
     (signal is additive over spins, so a subsample is an exact sub-problem)."""
     cfg = CONFIGS[index]

@@ -2,8 +2,8 @@
 __graft_entry__.build() with gcc against include/hom_abi.h and libhom.so)
 creates a plan, uploads the phantom, CSM and sequence, runs the whole block
 loop with hom_run and downloads the signal, in f64, f32 and mixed precision,
-on configuration 1's workload at every 500th spin (RF + 4 x 4750 samples, 8
-coils).  Its signal equals the Python ctypes path's bit for bit (same
+on the 350,000-spin 3D workload at every 500th spin (RF + 4 x 4750 samples,
+8 coils).  Its signal equals the Python ctypes path's bit for bit (same
 library, same fixed spin tile), and the f64 one matches the f64 oracle at
 the reference's tolerance."""
 import subprocess

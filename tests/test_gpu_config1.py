@@ -1,11 +1,11 @@
-"""Configuration 1 as specified: the f32 and f64 drop-in (coil modes "gemm"
+"""The f32 and f64 drop-in (coil modes "gemm"
 and "fused_tiled") on the 350,000-spin, 8-coil, 4 x 4750-sample 3D workload,
 run through the public simulate() with the reference demos' 32-node blocks
 (demo/SimulationDemoSupport.jl:114).
 
 Against the f64 oracle on every 250th spin (1400 spins, all four interleaves):
 the reference's GPU-vs-CPU tolerances (test/gpu/test_gpu_kernels.jl:31:
-isapprox f32 3e-5, f64 1e-11), and for f64 the north-star bar NRMSE <= 1e-12.
+isapprox f32 3e-5, f64 1e-11), and for f64 NRMSE <= 1e-12.
 f32 is compared with the oracle run on inputs rounded to f32, as the
 reference converts its inputs before the f32 path runs.
 

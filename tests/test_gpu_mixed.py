@@ -67,8 +67,8 @@ def test_mixed_shapes(nspins, ncoils, nsteps):
 
 
 def test_mixed_rf_interleaves_per_interleave_error():
-    """RF + spiral readout + idle to TR, 4 interleaves (configuration 1 geometry,
-    every 250th spin): NRMSE per interleave and overall."""
+    """RF + spiral readout + idle to TR, 4 interleaves (the 350,000-spin 3D
+    geometry, every 250th spin): NRMSE per interleave and overall."""
     require_gpu()
     obj, seq, cfg = make_config(1, spin_subsample=250)
     m = BlochHighOrder("1111")

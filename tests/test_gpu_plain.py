@@ -241,7 +241,7 @@ def test_block_length_invariance_and_run():
 
 
 def test_rf_and_precession_run_small_config():
-    """RF + spiral readout (configuration 0 geometry, 1/100 of its spins)."""
+    """RF + spiral readout (the 100 x 100 2D phantom, 1/100 of its spins)."""
     require_gpu()
     from hom_amd.workloads import make_config
     obj, seq, _ = make_config(0, spin_subsample=100)
