@@ -650,10 +650,14 @@ __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem
     const StageSrc s1 = stage_src<NMH>(tid + 256, L::kBopChunks, L::kAhoChunks, L::kRecChunks, o, seed_b);
     const StageSrc s2 = stage_src<NMH>(tid + 512, L::kBopChunks, L::kAhoChunks, L::kRecChunks, o, seed_b);
     const bool u0 = tid < L::kRecChunks, u1 = tid + 256 < L::kRecChunks, u2 = tid + 512 < L::kRecChunks;
-    const int wbase = (tid & ~63) * 16;
+    // The wave's LDS offset as a scalar (the copy's M0 base is then formed
+    // with scalar adds, not a readfirstlane per copy), and no lane mask for
+    // the copies that every thread makes (1.3 % faster at 32 coils).
+    const int wbase = __builtin_amdgcn_readfirstlane((tid & ~63) * 16);
     auto stage = [&](int64_t grp, unsigned char *dst) {
-        if (u0) copy16_to_lds(s0.p + grp * s0.stride, dst + wbase);
-        if (L::kStage > 1 && u1) copy16_to_lds(s1.p + grp * s1.stride, dst + wbase + 256 * 16);
+        if (L::kRecChunks >= 256 || u0) copy16_to_lds(s0.p + grp * s0.stride, dst + wbase);
+        if (L::kStage > 1 && (L::kRecChunks >= 512 || u1))
+            copy16_to_lds(s1.p + grp * s1.stride, dst + wbase + 256 * 16);
         if (L::kStage > 2 && u2) copy16_to_lds(s2.p + grp * s2.stride, dst + wbase + 512 * 16);
     };
 
