@@ -121,6 +121,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-subsample", type=int, default=128)
+    ap.add_argument("--no-cpu-reference", action="store_true",
+                    help="skip the reference-algorithm CPU timing (broadcast + zgemm restatement)")
+    ap.add_argument("--cpu-reference-subsample", type=int, default=8)
     ap.add_argument("--block-length", type=int, default=32)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-only", action="store_true")
@@ -321,6 +324,9 @@ def main():
                                 "sample": f"every {stride1}th spin ({len(sub1)} "
                                           f"spins) x all {total_adc} samples, 1 thread, {c1:.1f}s"}
         log(f"cpu oracle, 1 thread: {cpu['single_thread']['value']:.4e} spin-samples/s")
+        if not args.no_cpu_reference:
+            cpu["reference_algorithm"] = reference_algorithm_leg(
+                obj, seq, parts, method, threads, args.cpu_reference_subsample, cfg.n_adc)
         with Plan("mixed", len(sub), nc, seq.n_nodes, max_adc) as ps:
             ps.upload_phantom(sub)
             ps.upload_sequence(seq)
@@ -385,6 +391,38 @@ def main():
         "kernels": {"finalize_ms_avg": extra["finalize_ms_avg"], "prep_ms_avg": extra["prep_ms_avg"]},
     }
     print(json.dumps(out), flush=True)
+
+
+def reference_algorithm_leg(obj, seq, parts, method, threads, stride, per_readout,
+                            interleaves=2) -> dict:
+    """Times the reference's own CPU algorithm (oracle/broadcast.py: per-thread
+    contiguous spin parts, per-block broadcast temporaries of the fifteen
+    field terms, cumtrapz, complex exp and a zgemm coil combination) on every
+    ``stride``-th spin over the blocks of the first ``interleaves`` readouts,
+    an exact prefix of the run."""
+    from oracle import broadcast
+    from hom_amd.types import GAMMA
+    pf, pl, rf = (np.asarray(p) for p in parts)
+    adc_after = np.cumsum([int(np.count_nonzero(seq.adc[f + 1:l + 1])) for f, l in zip(pf, pl)])
+    nb = int(np.searchsorted(adc_after, per_readout * interleaves) + 1)
+    prefix = (pf[:nb], pl[:nb], rf[:nb])
+    samples = int(adc_after[nb - 1])
+    sub = obj[::stride]
+    log(f"reference CPU algorithm on {len(sub)} spins x {samples} samples, {threads} threads")
+    a = time.perf_counter()
+    broadcast.run(sub, seq, prefix, method.flags, GAMMA, n_threads=threads)
+    dt = time.perf_counter() - a
+    out = {"value": len(sub) * samples / dt, "unit": "spin-samples/s", "cores": threads,
+           "kind": "port",
+           "algorithm": "the reference CPU algorithm restated in numpy: contiguous spin parts per "
+                        "thread, per 32-node block the fifteen field terms as spins x nodes "
+                        "broadcast arrays, cumtrapz, complex exp, then one zgemm (OpenBLAS, one "
+                        "thread per worker) for the coil combination; numpy evaluates each "
+                        "broadcast expression through temporaries where Julia fuses it",
+           "sample": f"every {stride}th spin ({len(sub)} spins) x the first {samples} samples "
+                     f"({interleaves} of the run's readouts), f64, {threads} threads, {dt:.1f}s"}
+    log(f"reference CPU algorithm: {out['value']:.4e} spin-samples/s")
+    return out
 
 
 def serve_benchmark(args) -> dict:
