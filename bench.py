@@ -120,7 +120,7 @@ def main():
     ap.add_argument("--subsample", type=int, default=1, help="keep every k-th spin")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-subsample", type=int, default=128)
+    ap.add_argument("--cpu-subsample", type=int, default=40)
     ap.add_argument("--no-cpu-reference", action="store_true",
                     help="skip the reference-algorithm CPU timing (broadcast + zgemm restatement)")
     ap.add_argument("--cpu-reference-subsample", type=int, default=8)
