@@ -27,6 +27,7 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -466,6 +467,10 @@ struct hom_server {
     bool fatal = false;
     uint64_t epoch = 0;  // bumped whenever memory or the queue may let a request in
     bool blocked = false;  // worker: the queue head cannot be admitted until epoch changes
+    // Failure-handling tests: HOM_SERVE_TEST_FAIL_STEP=n (read at creation)
+    // makes step n end in a device error after its kernels have finished
+    // normally, so the stop path runs without any fault on the GPU.
+    int64_t test_fail_step = 0;
 };
 
 namespace {
@@ -925,11 +930,16 @@ void free_job_memory_locked(hom_server *s, Job &j) {
     s->stats.device_bytes_used = s->dev_arena.used;
 }
 
-void fail_active_locked(hom_server *s, const std::string &msg) {
+// After a device error or a failed step the server stops: every request that
+// has not finished fails -- the active ones release their arena memory, the
+// queued ones (which hold no arena memory yet) their resident pins and host
+// tables -- and every waiter wakes.
+void fail_unfinished_locked(hom_server *s, const std::string &msg) {
+    const double t = now_seconds();
     for (Job *j : s->active) {
         j->state = HOM_REQ_FAILED;
         j->err = msg;
-        j->t_done = now_seconds();
+        j->t_done = t;
         free_job_memory_locked(s, *j);
         if (j->host_off >= 0) s->host_arena.release(j->host_off, j->host_bytes);
         j->host_off = -1;
@@ -938,7 +948,17 @@ void fail_active_locked(hom_server *s, const std::string &msg) {
         s->stats.failed++;
     }
     s->active.clear();
+    for (Job *j : s->queue) {
+        j->state = HOM_REQ_FAILED;
+        j->err = msg;
+        j->t_done = t;
+        release_resident_locked(s, *j);
+        j->ts = TableSet{};
+        s->stats.failed++;
+    }
+    s->queue.clear();
     s->stats.active = 0;
+    s->stats.queued = 0;
     s->cv_done.notify_all();
 }
 
@@ -1036,11 +1056,13 @@ int step_once(hom_server *s, int *worked) {
     if (!rc) rc = hip_fail(s, hipStreamSynchronize(s->stream), "step");
     const double t1 = now_seconds();
     std::lock_guard<std::mutex> lk(s->mu);
+    if (!rc && s->test_fail_step > 0 && s->stats.steps + 1 == s->test_fail_step)
+        rc = fail_locked(s, HOM_ERR_DEVICE, "step: injected device error (HOM_SERVE_TEST_FAIL_STEP)");
     s->stats.steps++;
     s->stats.busy_seconds += t1 - t0;
     if (rc) {
         s->fatal = true;
-        fail_active_locked(s, s->err);
+        fail_unfinished_locked(s, s->err);
         return rc;
     }
     for (Job *j : finishing) {
@@ -1081,7 +1103,7 @@ int step_guarded(hom_server *s, int *worked) {
     std::lock_guard<std::mutex> lk(s->mu);
     s->fatal = true;
     s->err = msg;
-    fail_active_locked(s, msg);
+    fail_unfinished_locked(s, msg);
     s->cv_done.notify_all();
     return code;
 }
@@ -1226,6 +1248,7 @@ static int hom_server_create_impl(hom_server **out, const hom_server_config *con
     std::memset(s->h_desc, 0, sizeof(StepDesc));
     s->dev_arena.init(c.device_bytes);
     s->host_arena.init(c.host_bytes);
+    if (const char *v = std::getenv("HOM_SERVE_TEST_FAIL_STEP")) s->test_fail_step = std::atoll(v);
     hom_server *raw = s.release();
     if (c.threaded) raw->worker = std::thread(worker_main, raw);
     *out = raw;
@@ -1545,7 +1568,7 @@ int hom_server_wait(hom_server *s, int64_t rid, double timeout_s, int *state) {
     if (it == s->jobs.end()) return fail_locked(s, HOM_ERR_ARGUMENT, "Unknown request id.");
     Job *j = it->second.get();
     auto ready = [&] {
-        return j->state == HOM_REQ_DONE || j->state == HOM_REQ_FAILED ||
+        return j->state == HOM_REQ_DONE || j->state == HOM_REQ_FAILED || s->fatal ||
                (j->state == HOM_REQ_QUEUED && s->blocked && s->active.empty());
     };
     if (timeout_s < 0) s->cv_done.wait(lk, ready);

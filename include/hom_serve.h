@@ -28,6 +28,14 @@
  * Thread safety: every entry point may be called from any thread; the
  * server serializes them internally.  With threaded = 1 a worker thread runs
  * steps whenever work is queued; otherwise the caller drives hom_server_step.
+ * hom_server_destroy is the exception: no other call on the server may be in
+ * progress or follow it (it stops the worker, abandons unfinished requests
+ * and frees everything).
+ *
+ * Errors: a device error, or a host allocation failure inside a step, stops
+ * the server.  Every request that has not finished (active or queued) turns
+ * HOM_REQ_FAILED, waiters return, and later submissions and steps fail with
+ * HOM_ERR_DEVICE; hom_server_last_error holds the cause.
  */
 #ifndef HOM_SERVE_H
 #define HOM_SERVE_H

@@ -299,3 +299,36 @@ def test_serve_block_description_sequence():
     assert sig.shape == ref.shape == (1200, 8)
     e = nrmse(sig, ref)
     assert e <= NRMSE_BAR, f"NRMSE {e:.3e}"
+
+
+def test_device_error_fails_queued_requests(inputs, monkeypatch):
+    """A step that ends in a device error stops the server: the active request
+    and the queued ones fail (queued requests used to stay queued, so a waiter
+    on one never returned), and later submissions and steps report the error.
+    The error is raised on the host after step 2's kernels have finished
+    normally (HOM_SERVE_TEST_FAIL_STEP, read when the server is created)."""
+    require_gpu()
+    obj, seq = inputs
+    method = BlochHighOrder("1111")
+    monkeypatch.setenv("HOM_SERVE_TEST_FAIL_STEP", "2")
+    srv = SimulationServer(memory_bytes=512 * MB, result_bytes=64 * MB, max_active=1,
+                           step_budget=1 << 23, threaded=False)
+    monkeypatch.delenv("HOM_SERVE_TEST_FAIL_STEP")
+    with srv:
+        ph = srv.put_phantom(sub_phantom(obj, 0, 20_000, 8))
+        sq = srv.put_sequence(seq)
+        reqs = [srv.submit(ph, sq, method) for _ in range(3)]
+        print_case(f"injected failure at step 2, {len(reqs)} requests of 20000 spins x 8 coils")
+        assert srv.step()                       # step 1: admits the first request
+        assert srv.srv.status(reqs[0].id) == "active"
+        assert srv.srv.status(reqs[1].id) == "queued"
+        with pytest.raises(HomDeviceError, match="injected"):
+            srv.step()                          # step 2 fails
+        assert [srv.srv.status(r.id) for r in reqs] == ["failed"] * 3
+        assert [srv.srv.wait(r.id, 5.0) for r in reqs] == ["failed"] * 3
+        st = srv.stats()
+        assert st["failed"] == 3 and st["queued"] == 0 and st["active"] == 0
+        with pytest.raises(HomDeviceError):
+            srv.submit(ph, sq, method)
+        with pytest.raises(HomDeviceError):
+            srv.drain()
