@@ -1564,15 +1564,22 @@ int hom_server_wait(hom_server *s, int64_t rid, double timeout_s, int *state) {
         }
     }
     std::unique_lock<std::mutex> lk(s->mu);
-    auto it = s->jobs.find(rid);
-    if (it == s->jobs.end()) return fail_locked(s, HOM_ERR_ARGUMENT, "Unknown request id.");
-    Job *j = it->second.get();
+    // The request is looked up again on every wake-up: another thread may
+    // fetch its result (which forgets it) while this one waits.
+    auto find = [&]() -> Job * {
+        auto it = s->jobs.find(rid);
+        return it == s->jobs.end() ? nullptr : it->second.get();
+    };
+    if (!find()) return fail_locked(s, HOM_ERR_ARGUMENT, "Unknown request id.");
     auto ready = [&] {
-        return j->state == HOM_REQ_DONE || j->state == HOM_REQ_FAILED || s->fatal ||
+        const Job *j = find();
+        return !j || j->state == HOM_REQ_DONE || j->state == HOM_REQ_FAILED || s->fatal ||
                (j->state == HOM_REQ_QUEUED && s->blocked && s->active.empty());
     };
     if (timeout_s < 0) s->cv_done.wait(lk, ready);
     else s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready);
+    const Job *j = find();
+    if (!j) return fail_locked(s, HOM_ERR_ARGUMENT, "The request's result was fetched by another call.");
     if (state) *state = j->state;
     if (j->state == HOM_REQ_QUEUED && s->blocked && s->active.empty())
         return fail_locked(s, HOM_ERR_MEMORY,

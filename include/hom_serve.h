@@ -122,7 +122,9 @@ int hom_server_footprint(hom_server *server, const hom_request *req, int64_t *by
 int hom_server_step(hom_server *server, int *worked);
 /* Runs steps (or waits for the worker) until nothing is queued or active. */
 int hom_server_drain(hom_server *server);
-/* Waits up to timeout_s (< 0: no limit) for a request; *state gets its state. */
+/* Waits up to timeout_s (< 0: no limit) for a request; *state gets its state.
+ * HOM_ERR_ARGUMENT when the id is unknown, or when another thread fetched the
+ * request's result (which forgets it) during the wait. */
 int hom_server_wait(hom_server *server, int64_t request_id, double timeout_s, int *state);
 int hom_server_status(hom_server *server, int64_t request_id, int *state);
 /* Rows (total ADC samples) and columns (coils) of a request's signal. */
