@@ -187,3 +187,41 @@ def test_mixed_deterministic_and_matches_f64_plan():
     assert np.array_equal(a, b)
     f, _, _ = run_mixed(obj, seq, parts, m, precision="f64")
     assert nrmse(a, f) <= NRMSE_BAR
+
+
+def scaled_gradients(seq, factor):
+    """A copy of seq with the first-order waveforms (h1..h3 and the nominal
+    gradients) scaled by factor."""
+    import copy
+    s = copy.deepcopy(seq)
+    s.h = np.array(s.h, copy=True)
+    s.h[1:4] *= factor
+    s.gx, s.gy, s.gz = s.gx * factor, s.gy * factor, s.gz * factor
+    return s
+
+
+@pytest.mark.parametrize("factor,fp64_segments", [(4.0, 0), (9.0, 4)])
+def test_mixed_strong_gradients_stay_split(factor, fp64_segments):
+    """Strong gradients: the 350,000-spin, 8-coil, 4 x 4750-sample spiral
+    workload (every 500th spin) with its gradients scaled 4x keeps all four
+    readouts on the split-phase body (first-order offsets up to about 8
+    revolutions per term within 8 samples); scaled 9x (about 18 revolutions)
+    they run the all-FP64 body.  Both against the f64 oracle per readout."""
+    require_gpu()
+    obj, seq, cfg = make_config(1, spin_subsample=500)
+    seq = scaled_gradients(seq, factor)
+    m = BlochHighOrder("1111")
+    parts = get_sim_ranges(seq, 32, 32)
+    print_case("mixed-strong-gradients", factor, len(obj))
+    ref, _, _ = oracle.run(obj, seq, parts, m.flags, GAMMA, n_threads=8)
+    nadc = max(seq.block_adc(a, b) for a, b in zip(parts[0], parts[1]))
+    with Plan("mixed", len(obj), obj.n_coils, seq.n_nodes, nadc) as plan:
+        plan.upload_phantom(obj)
+        plan.upload_sequence(seq)
+        plan.run(parts, m)
+        sig = plan.download_signal()
+        segments, fp64 = plan.mixed_stats()
+    assert (segments, fp64) == (cfg.n_interleaves, fp64_segments)
+    n = cfg.n_adc
+    for i in range(cfg.n_interleaves):
+        assert nrmse(sig[i * n:(i + 1) * n], ref[i * n:(i + 1) * n]) <= NRMSE_BAR
