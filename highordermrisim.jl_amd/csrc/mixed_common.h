@@ -63,9 +63,17 @@ constexpr int kOpScale = 14;
 // B = dKr_j * 2^(kHOOut - a_j); the FP32 product sum is offset * 2^kHOOut.
 constexpr int kHOA = 10;
 constexpr int kHOOut = 20;
-// |B| <= 2^9 with |A| <= 2^10 bounds every higher-order offset by 0.5
-// revolutions.  Segments with larger offsets use the all-FP64 phase body.
-constexpr float kHOBMax = 512.0f;
+// |B| <= 2^13 with |A| <= 2^10 bounds every higher-order offset by 8
+// revolutions within its 256-sample anchor block; segments with larger
+// offsets use the all-FP64 phase body.  As for the first order below, the
+// bound only selects the body: on the 1M-spin, 32-coil, 16 x 8000-sample
+// spiral with its 2nd- and 3rd-order fields scaled 1x / 10x / 30x / 100x the
+// split path measured NRMSE 0.88 / 0.94 / 0.92 / 0.97e-6 against the f64
+// oracle, where a bound of 2^9 (0.5 revolution) sent 10x and more to the
+// all-FP64 body (1.45x the kernel time).  With the first-order bound, a
+// phase stays below 12 x 8 + 4 x 16 + 1 = 161 revolutions, inside the
+// 256-revolution input range of v_sin_f32 / v_cos_f32.
+constexpr float kHOBMax = 8192.0f;
 // First-order offsets: A = b_j * 2^a_j with max |A| in (2^13, 2^14], B =
 // dKr_j * 2^(kHOOut - a_j) (the same product scale, one accumulator), split
 // three ways; |B| <= 2^10 bounds each term by 16 revolutions (a spin 0.1 m

@@ -225,3 +225,33 @@ def test_mixed_strong_gradients_stay_split(factor, fp64_segments):
     n = cfg.n_adc
     for i in range(cfg.n_interleaves):
         assert nrmse(sig[i * n:(i + 1) * n], ref[i * n:(i + 1) * n]) <= NRMSE_BAR
+
+
+@pytest.mark.parametrize("factor,fp64_segments", [(30.0, 0), (300.0, 4)])
+def test_mixed_strong_higher_order_fields_stay_split(factor, fp64_segments):
+    """Strong 2nd- and 3rd-order fields: the 350,000-spin, 8-coil,
+    4 x 4750-sample workload (every 500th spin) with h4..h15 scaled 30x keeps
+    all four readouts on the split-phase body (higher-order offsets of a few
+    revolutions within a 256-sample anchor block); scaled 300x they run the
+    all-FP64 body.  Both against the f64 oracle per readout."""
+    import copy
+    require_gpu()
+    obj, seq, cfg = make_config(1, spin_subsample=500)
+    seq = copy.deepcopy(seq)
+    seq.h = np.array(seq.h, copy=True)
+    seq.h[4:16] *= factor
+    m = BlochHighOrder("1111")
+    parts = get_sim_ranges(seq, 32, 32)
+    print_case("mixed-strong-higher-order", factor, len(obj))
+    ref, _, _ = oracle.run(obj, seq, parts, m.flags, GAMMA, n_threads=8)
+    nadc = max(seq.block_adc(a, b) for a, b in zip(parts[0], parts[1]))
+    with Plan("mixed", len(obj), obj.n_coils, seq.n_nodes, nadc) as plan:
+        plan.upload_phantom(obj)
+        plan.upload_sequence(seq)
+        plan.run(parts, m)
+        sig = plan.download_signal()
+        segments, fp64 = plan.mixed_stats()
+    assert (segments, fp64) == (cfg.n_interleaves, fp64_segments)
+    n = cfg.n_adc
+    for i in range(cfg.n_interleaves):
+        assert nrmse(sig[i * n:(i + 1) * n], ref[i * n:(i + 1) * n]) <= NRMSE_BAR
