@@ -177,7 +177,31 @@ struct TableSet {
 
 TermList make_terms(int flags);
 PhaseSplit make_split(int flags, const double *basis_max);
-// Segments (consecutive precession parts fused) and their integrated tables.
+// Segments (consecutive precession parts fused) and their integrated tables,
+// in two steps: plan() lists the segments with their sizes and table offsets
+// and sizes every table; fill(i) computes precession segment i's entries (and
+// decides its phase body), so a caller can fill segment i + 1 on the host
+// while the device runs segment i.  The all-FP64 table (kr) grows by each
+// falling-back segment as it is filled, within kr_bound() entries.
+class TableBuilder {
+  public:
+    TableBuilder(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, int NI, int flags,
+                 std::vector<Segment> &segs, TableSet &ts)
+        : sq_(sq), tl_(tl), ps_(ps), NI_(NI), flags_(flags), segs_(segs), ts_(ts) {}
+    void plan(const int64_t *pf, const int64_t *pl, const uint8_t *rf, int64_t np);
+    void fill(size_t i);
+    int64_t kr_bound() const { return kr_bound_; }
+
+  private:
+    const SeqHost &sq_;
+    const TermList &tl_;
+    const PhaseSplit &ps_;
+    int NI_, flags_;
+    std::vector<Segment> &segs_;
+    TableSet &ts_;
+    int64_t kr_bound_ = 0;
+};
+// Both steps for every segment.
 void build_tables(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, int NI, int flags,
                   const int64_t *pf, const int64_t *pl, const uint8_t *rf, int64_t np,
                   std::vector<Segment> &segs, TableSet &ts);

@@ -291,16 +291,26 @@ hipError_t launch_main(int ncb, bool fallback, int nmh, int NI, hipStream_t st, 
     }
 }
 
+// Replaces *dst with a device buffer of n elements (none for n = 0).
 template <typename T>
-int dev_upload(hom_plan *p, T **dst, const std::vector<T> &src) {
+int dev_alloc(hom_plan *p, T **dst, size_t n) {
     if (*dst) (void)hipFree(*dst);
     *dst = nullptr;
-    if (src.empty()) return HOM_OK;
-    if (hom::dev_malloc((void **)dst, sizeof(T) * src.size()) != hipSuccess) {
+    if (n == 0) return HOM_OK;
+    if (hom::dev_malloc((void **)dst, sizeof(T) * n) != hipSuccess) {
         *dst = nullptr;
         return set_error(p, HOM_ERR_MEMORY, "device allocation failed for mixed-path tables");
     }
-    return check_hip(p, hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice),
+    return HOM_OK;
+}
+
+// Copies elements [off, off + n) of a host table to the same range of its
+// device buffer (synchronous; the plan's stream may still run earlier
+// segments, which read other ranges).
+template <typename T>
+int upload_range(hom_plan *p, T *dev, const std::vector<T> &host, int64_t off, int64_t n) {
+    if (n <= 0) return HOM_OK;
+    return check_hip(p, hipMemcpy(dev + off, host.data() + off, sizeof(T) * (size_t)n, hipMemcpyHostToDevice),
                      "hipMemcpy");
 }
 
@@ -410,11 +420,62 @@ double parallel_max(int64_t n, F &&fill) {
     return *std::max_element(m.begin(), m.end());
 }
 
-void build_tables(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, int NI, int flags,
-                  const int64_t *pf, const int64_t *pl, const uint8_t *rf, int64_t np,
-                  std::vector<Segment> &segs, TableSet &ts) {
-    segs.clear();
-    ts = TableSet{};
+void TableBuilder::plan(const int64_t *pf, const int64_t *pl, const uint8_t *rf, int64_t np) {
+    segs_.clear();
+    ts_ = TableSet{};
+    const int NMH = ps_.nmh, NHO = ps_.nho;
+    int64_t row = 0, b = 0, tiles = 0, nab = 0, nprec = 0;
+    kr_bound_ = 0;
+    while (b < np) {
+        Segment s;
+        if (rf[b]) {
+            s.rf = true;
+            s.first = pf[b];
+            s.last = pl[b];
+            ++b;
+        } else {
+            int64_t e = b;
+            while (e + 1 < np && !rf[e + 1]) ++e;
+            s.first = pf[b];
+            s.last = pl[e];
+            b = e + 1;
+        }
+        for (int64_t n = s.first + 1; n <= s.last; ++n) s.nadc += sq_.adc[(size_t)n] ? 1 : 0;
+        s.row0 = row;
+        row += s.nadc;
+        if (!s.rf) {
+            s.tiles = (s.nadc + 15) / 16;
+            s.nsb = (s.tiles + kBlockTiles - 1) / kBlockTiles;
+            s.nab = (s.tiles + kAnchorTiles - 1) / kAnchorTiles;
+            s.kref_off = tiles * kRefsPerTile * 4;
+            s.bho_off = tiles * NMH * 64;
+            s.s_off = tiles * 16;
+            s.kend_off = nprec * (tl_.n + 1);
+            s.anc_off = nab * NHO;
+            tiles += s.tiles;
+            nab += s.nab;
+            ++nprec;
+            kr_bound_ += s.tiles * NI_ * 64;
+        }
+        segs_.push_back(s);
+    }
+    // Every table sized once (growing them segment by segment copies
+    // everything accumulated so far at each reallocation).
+    ts_.kref.assign((size_t)(tiles * kRefsPerTile * 4), 0.0);
+    ts_.bho.assign((size_t)(tiles * NMH * 64), half8_t{});
+    ts_.tk.assign((size_t)(tiles * 16), 0.0f);
+    ts_.kr0.assign((size_t)(tiles * 16), 0.0);
+    ts_.anc.assign((size_t)(nab * NHO), 0.0);
+    ts_.kend.assign((size_t)(nprec * (tl_.n + 1)), 0.0);
+}
+
+void TableBuilder::fill(size_t i) {
+    Segment &s = segs_[i];
+    if (s.rf) return;
+    const SeqHost &sq = sq_;
+    const TermList &tl = tl_;
+    const PhaseSplit &ps = ps_;
+    const int NI = NI_, flags = flags_;
     const int NMH = ps.nmh, NHO = ps.nho, N1 = ps.n1;
     const int64_t nn = sq.nn;
     const bool ho1 = flags & HOM_HO1, ho0 = flags & HOM_HO0;
@@ -443,161 +504,111 @@ void build_tables(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, i
     double bscale[12], bscale1[4];
     for (int q = 0; q < NHO; ++q) bscale[q] = std::ldexp(1.0, kHOOut - ps.aexp[q]);
     for (int q = 0; q < N1; ++q) bscale1[q] = std::ldexp(1.0, kHOOut - ps.aexp1[q]);
-    auto &kr = ts.kr, &kref = ts.kref, &kr0 = ts.kr0, &kend = ts.kend, &anc = ts.anc;
-    auto &bho = ts.bho;
-    auto &tk = ts.tk;
-    {
-        // Size every table once: growing them segment by segment copies
-        // everything accumulated so far at each reallocation.
-        int64_t tiles = 0, nab = 0;
-        for (int64_t b = 0; b < np;) {
-            if (rf[b]) {
-                ++b;
-                continue;
-            }
-            int64_t e = b, nadc = 0;
-            while (e + 1 < np && !rf[e + 1]) ++e;
-            for (int64_t n = pf[b] + 1; n <= pl[e]; ++n) nadc += sq.adc[(size_t)n] ? 1 : 0;
-            tiles += (nadc + 15) / 16;
-            nab += ((nadc + 15) / 16 + kAnchorTiles - 1) / kAnchorTiles;
-            b = e + 1;
-        }
-        kref.reserve((size_t)(tiles * kRefsPerTile * 4));
-        bho.reserve((size_t)(tiles * NMH * 64));
-        tk.reserve((size_t)(tiles * 16));
-        kr0.reserve((size_t)(tiles * 16));
-        anc.reserve((size_t)(nab * NHO));
+    auto &kr = ts_.kr, &kref = ts_.kref, &kr0 = ts_.kr0, &kend = ts_.kend, &anc = ts_.anc;
+    auto &bho = ts_.bho;
+    auto &tk = ts_.tk;
+    // Phase 1 (sequential): the trapezoid integrals at every acquired
+    // node (Kr of each term, the time, the zeroth order).
+    const int nt = tl.n;
+    std::vector<double> KrS((size_t)(s.nadc * nt)), tS((size_t)s.nadc), i0S((size_t)s.nadc);
+    std::vector<double> I(nt, 0.0);
+    double I0 = 0.0, t = 0.0;
+    int64_t k = 0;
+    for (int64_t n = s.first + 1; n <= s.last; ++n) {
+        const double dt = sq.dt[(size_t)(n - 1)];
+        for (int q = 0; q < nt; ++q)
+            if (wave[q]) I[q] += (wave[q][n - 1] + wave[q][n]) * (dt / 2.0);
+        if (ho0) I0 += (h0[n - 1] + h0[n]) * (dt / 2.0);
+        t += dt;
+        if (!sq.adc[(size_t)n]) continue;
+        double *kr_k = &KrS[(size_t)(k * nt)];
+        for (int q = 0; q < nt; ++q) kr_k[q] = tl.code[q] == kTermDW ? -t / kTwoPi : -g * I[q];
+        tS[(size_t)k] = t;
+        i0S[(size_t)k] = I0;
+        ++k;
     }
-    int64_t row = 0, b = 0;
-    while (b < np) {
-        Segment s;
-        if (rf[b]) {
-            s.rf = true;
-            s.first = pf[b];
-            s.last = pl[b];
-            for (int64_t n = s.first + 1; n <= s.last; ++n) s.nadc += sq.adc[(size_t)n] ? 1 : 0;
-            s.row0 = row;
-            row += s.nadc;
-            segs.push_back(s);
-            ++b;
-            continue;
+    // higher-order anchors: the values at the first sample of each 256-sample block
+    for (int64_t ab = 0; ab < s.nab; ++ab)
+        for (int q = 0; q < NHO; ++q)
+            anc[(size_t)(s.anc_off + ab * NHO + q)] = KrS[(size_t)(ab * 16 * kAnchorTiles * nt + posho[q])];
+    // first-order anchors: the values at each tile's reference sample
+    for (int64_t tau = 0; tau < s.tiles; ++tau)
+        for (int h = 0; h < kRefsPerTile; ++h) {
+            const int64_t r = ref_sample(tau, h, s.nadc);
+            for (int q = 0; q < 4; ++q)
+                kref[(size_t)(s.kref_off + (tau * kRefsPerTile + h) * 4 + q)] =
+                    pos64[q] >= 0 ? KrS[(size_t)(r * nt + pos64[q])] : 0.0;
         }
-        int64_t e = b;
-        while (e + 1 < np && !rf[e + 1]) ++e;
-        s.first = pf[b];
-        s.last = pl[e];
-        b = e + 1;
-        for (int64_t n = s.first + 1; n <= s.last; ++n) s.nadc += sq.adc[(size_t)n] ? 1 : 0;
-        s.row0 = row;
-        row += s.nadc;
-        s.tiles = (s.nadc + 15) / 16;
-        s.nsb = (s.tiles + kBlockTiles - 1) / kBlockTiles;
-        s.nab = (s.tiles + kAnchorTiles - 1) / kAnchorTiles;
-        s.kref_off = (int64_t)kref.size();
-        s.bho_off = (int64_t)bho.size();
-        s.s_off = (int64_t)tk.size();
-        s.kend_off = (int64_t)kend.size();
-        s.anc_off = (int64_t)anc.size();
-        kref.resize(kref.size() + (size_t)(s.tiles * kRefsPerTile * 4), 0.0);
-        bho.resize(bho.size() + (size_t)(s.tiles * NMH * 64), half8_t{});
-        tk.resize(tk.size() + (size_t)(s.tiles * 16), 0.0f);
-        kr0.resize(kr0.size() + (size_t)(s.tiles * 16), 0.0);
-        anc.resize(anc.size() + (size_t)(s.nab * NHO), 0.0);
-        // Phase 1 (sequential): the trapezoid integrals at every acquired
-        // node (Kr of each term, the time, the zeroth order).
-        const int nt = tl.n;
-        std::vector<double> KrS((size_t)(s.nadc * nt)), tS((size_t)s.nadc), i0S((size_t)s.nadc);
-        std::vector<double> I(nt, 0.0);
-        double I0 = 0.0, t = 0.0;
-        int64_t k = 0;
-        for (int64_t n = s.first + 1; n <= s.last; ++n) {
-            const double dt = sq.dt[(size_t)(n - 1)];
+    // Phase 2 (samples independent, split over threads): FP64 and
+    // split-FP16 table entries, and the largest scaled offset.
+    _Float16 *const bho16 = reinterpret_cast<_Float16 *>(bho.data());
+    // The largest |B| is reported relative to its bound: higher orders
+    // |B| / kHOBMax, first order |B| / kFOBMax (the segment falls back to
+    // the all-FP64 body above 1).
+    auto fill_range = [&](int64_t k0, int64_t k1, double &bm_out) {
+        double bm = 0.0;  // local: the threads' results share a cache line
+        for (int64_t kk = k0; kk < k1; ++kk) {
+            const double *kr_k = &KrS[(size_t)(kk * nt)];
+            const int64_t tau = kk / 16, j = kk % 16, ab = kk / (16 * kAnchorTiles);
+            const double *kr_ref = &KrS[(size_t)(ref_sample(tau, (int)(j / kRefHalf), s.nadc) * nt)];
+            // first-order slot 3 NHO + p N1 + term holds part fo_bpart(p)
+            // of the three-way split offset from the tile reference
+            for (int term = 0; term < N1; ++term) {
+                const double bval = (kr_k[pos1[term]] - kr_ref[pos1[term]]) * bscale1[term];
+                bm = std::max(bm, std::fabs(bval) / kFOBMax);
+                _Float16 parts[3];
+                host_split16x3(bval, parts);
+                for (int prod = 0; prod < kFOParts; ++prod) {
+                    const int slot = 3 * NHO + prod * N1 + term, m = slot / 32, sl = slot % 32;
+                    bho16[(size_t)(s.bho_off + (tau * NMH + m) * 64 + 16 * (sl / 8) + j) * 8 + sl % 8] =
+                        parts[fo_bpart(prod)];
+                }
+            }
+            // slots term, NHO + term, 2 NHO + term hold the B-side hi, hi,
+            // lo of the term's split offset (the A side holds hi, lo, hi)
+            for (int term = 0; term < NHO; ++term) {
+                const double bval =
+                    (kr_k[posho[term]] - anc[(size_t)(s.anc_off + ab * NHO + term)]) * bscale[term];
+                bm = std::max(bm, std::fabs(bval) / kHOBMax);
+                _Float16 hi, lo;
+                host_split16(bval, hi, lo);
+                for (int kind = 0; kind < 3; ++kind) {
+                    const int slot = kind * NHO + term, m = slot / 32, sl = slot % 32;
+                    // element sl % 8 of the 8-half chunk, stored as a scalar
+                    bho16[(size_t)(s.bho_off + (tau * NMH + m) * 64 + 16 * (sl / 8) + j) * 8 + sl % 8] =
+                        kind == 2 ? lo : hi;
+                }
+            }
+            tk[(size_t)(s.s_off + kk)] = (float)tS[(size_t)kk];
+            kr0[(size_t)(s.s_off + kk)] = -g * i0S[(size_t)kk];
+        }
+        bm_out = bm;
+    };
+    const double bmax = parallel_max(s.nadc, fill_range);
+    s.fallback = !(bmax <= 1.0);
+    if (s.fallback) {
+        // All-FP64 table of every term, only for segments that need it.
+        s.kr_off = (int64_t)kr.size();
+        kr.resize(kr.size() + (size_t)(s.tiles * NI * 64), 0.0);
+        for (int64_t kk = 0; kk < s.nadc; ++kk) {
+            const int64_t tau = kk / 16, j = kk % 16;
             for (int q = 0; q < nt; ++q)
-                if (wave[q]) I[q] += (wave[q][n - 1] + wave[q][n]) * (dt / 2.0);
-            if (ho0) I0 += (h0[n - 1] + h0[n]) * (dt / 2.0);
-            t += dt;
-            if (!sq.adc[(size_t)n]) continue;
-            double *kr_k = &KrS[(size_t)(k * nt)];
-            for (int q = 0; q < nt; ++q) kr_k[q] = tl.code[q] == kTermDW ? -t / kTwoPi : -g * I[q];
-            tS[(size_t)k] = t;
-            i0S[(size_t)k] = I0;
-            ++k;
+                kr[(size_t)(s.kr_off + (tau * NI + q / 4) * 64 + 16 * (q % 4) + j)] =
+                    KrS[(size_t)(kk * nt + q)];
         }
-        // higher-order anchors: the values at the first sample of each 256-sample block
-        for (int64_t ab = 0; ab < s.nab; ++ab)
-            for (int q = 0; q < NHO; ++q)
-                anc[(size_t)(s.anc_off + ab * NHO + q)] = KrS[(size_t)(ab * 16 * kAnchorTiles * nt + posho[q])];
-        // first-order anchors: the values at each tile's reference sample
-        for (int64_t tau = 0; tau < s.tiles; ++tau)
-            for (int h = 0; h < kRefsPerTile; ++h) {
-                const int64_t r = ref_sample(tau, h, s.nadc);
-                for (int q = 0; q < 4; ++q)
-                    kref[(size_t)(s.kref_off + (tau * kRefsPerTile + h) * 4 + q)] =
-                        pos64[q] >= 0 ? KrS[(size_t)(r * nt + pos64[q])] : 0.0;
-            }
-        // Phase 2 (samples independent, split over threads): FP64 and
-        // split-FP16 table entries, and the largest scaled offset.
-        _Float16 *const bho16 = reinterpret_cast<_Float16 *>(bho.data());
-        // The largest |B| is reported relative to its bound: higher orders
-        // |B| / kHOBMax, first order |B| / kFOBMax (the segment falls back to
-        // the all-FP64 body above 1).
-        auto fill = [&](int64_t k0, int64_t k1, double &bm_out) {
-            double bm = 0.0;  // local: the threads' results share a cache line
-            for (int64_t kk = k0; kk < k1; ++kk) {
-                const double *kr_k = &KrS[(size_t)(kk * nt)];
-                const int64_t tau = kk / 16, j = kk % 16, ab = kk / (16 * kAnchorTiles);
-                const double *kr_ref = &KrS[(size_t)(ref_sample(tau, (int)(j / kRefHalf), s.nadc) * nt)];
-                // first-order slot 3 NHO + p N1 + term holds part fo_bpart(p)
-                // of the three-way split offset from the tile reference
-                for (int term = 0; term < N1; ++term) {
-                    const double bval = (kr_k[pos1[term]] - kr_ref[pos1[term]]) * bscale1[term];
-                    bm = std::max(bm, std::fabs(bval) / kFOBMax);
-                    _Float16 parts[3];
-                    host_split16x3(bval, parts);
-                    for (int prod = 0; prod < kFOParts; ++prod) {
-                        const int slot = 3 * NHO + prod * N1 + term, m = slot / 32, sl = slot % 32;
-                        bho16[(size_t)(s.bho_off + (tau * NMH + m) * 64 + 16 * (sl / 8) + j) * 8 + sl % 8] =
-                            parts[fo_bpart(prod)];
-                    }
-                }
-                // slots term, NHO + term, 2 NHO + term hold the B-side hi, hi,
-                // lo of the term's split offset (the A side holds hi, lo, hi)
-                for (int term = 0; term < NHO; ++term) {
-                    const double bval =
-                        (kr_k[posho[term]] - anc[(size_t)(s.anc_off + ab * NHO + term)]) * bscale[term];
-                    bm = std::max(bm, std::fabs(bval) / kHOBMax);
-                    _Float16 hi, lo;
-                    host_split16(bval, hi, lo);
-                    for (int kind = 0; kind < 3; ++kind) {
-                        const int slot = kind * NHO + term, m = slot / 32, sl = slot % 32;
-                        // element sl % 8 of the 8-half chunk, stored as a scalar
-                        bho16[(size_t)(s.bho_off + (tau * NMH + m) * 64 + 16 * (sl / 8) + j) * 8 + sl % 8] =
-                            kind == 2 ? lo : hi;
-                    }
-                }
-                tk[(size_t)(s.s_off + kk)] = (float)tS[(size_t)kk];
-                kr0[(size_t)(s.s_off + kk)] = -g * i0S[(size_t)kk];
-            }
-            bm_out = bm;
-        };
-        const double bmax = parallel_max(s.nadc, fill);
-        s.fallback = !(bmax <= 1.0);
-        if (s.fallback) {
-            // All-FP64 table of every term, only for segments that need it.
-            s.kr_off = (int64_t)kr.size();
-            kr.resize(kr.size() + (size_t)(s.tiles * NI * 64), 0.0);
-            for (int64_t kk = 0; kk < s.nadc; ++kk) {
-                const int64_t tau = kk / 16, j = kk % 16;
-                for (int q = 0; q < nt; ++q)
-                    kr[(size_t)(s.kr_off + (tau * NI + q / 4) * 64 + 16 * (q % 4) + j)] =
-                        KrS[(size_t)(kk * nt + q)];
-            }
-        }
-        for (int q = 0; q < tl.n; ++q) kend.push_back(tl.code[q] == kTermDW ? -t / kTwoPi : -g * I[q]);
-        kend.push_back(-g * I0);
-        s.T = t;
-        segs.push_back(s);
     }
+    for (int q = 0; q < tl.n; ++q)
+        kend[(size_t)(s.kend_off + q)] = tl.code[q] == kTermDW ? -t / kTwoPi : -g * I[q];
+    kend[(size_t)(s.kend_off + tl.n)] = -g * I0;
+    s.T = t;
+}
+
+void build_tables(const SeqHost &sq, const TermList &tl, const PhaseSplit &ps, int NI, int flags,
+                  const int64_t *pf, const int64_t *pl, const uint8_t *rf, int64_t np,
+                  std::vector<Segment> &segs, TableSet &ts) {
+    TableBuilder tb(sq, tl, ps, NI, flags, segs, ts);
+    tb.plan(pf, pl, rf, np);
+    for (size_t i = 0; i < segs.size(); ++i) tb.fill(i);
 }
 
 void mixed_free(hom_plan *p) {
@@ -613,6 +624,23 @@ void mixed_free(hom_plan *p) {
 }
 
 namespace {
+// Device copy of one precession segment's table entries; a segment that
+// falls back also copies its all-FP64 table, into a buffer sized on the
+// first fallback for every segment's (kr_bound entries).
+int upload_segment_tables(hom_plan *p, MixedWorkspace &W, const TableSet &ts, const Segment &s,
+                          int64_t kr_bound) {
+    const int64_t nho = W.split.nho, nmh = W.split.nmh;
+    int rc = upload_range(p, W.kref, ts.kref, s.kref_off, s.tiles * kRefsPerTile * 4);
+    rc = rc ? rc : upload_range(p, W.bho, ts.bho, s.bho_off, s.tiles * nmh * 64);
+    rc = rc ? rc : upload_range(p, W.tk, ts.tk, s.s_off, s.tiles * 16);
+    rc = rc ? rc : upload_range(p, W.kr0, ts.kr0, s.s_off, s.tiles * 16);
+    rc = rc ? rc : upload_range(p, W.anc, ts.anc, s.anc_off, s.nab * nho);
+    rc = rc ? rc : upload_range(p, W.kend, ts.kend, s.kend_off, (int64_t)W.terms.n + 1);
+    if (rc || !s.fallback) return rc;
+    if (!W.kr && (rc = dev_alloc(p, &W.kr, (size_t)kr_bound))) return rc;
+    return upload_range(p, W.kr, ts.kr, s.kr_off, s.tiles * W.NI * 64);
+}
+
 // Spin tile of one segment of nsb sample blocks: the fixed tile, or the
 // largest power of two up to the bound whose work items fill the device's
 // workgroup slots (two per CU) in whole waves best.  A smaller tile is taken
@@ -709,29 +737,35 @@ int mixed_exec(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t 
         key.push_back(pl[b]);
         key.push_back(rf[b]);
     }
-    if (W.table_flags != mflags || W.table_parts != key) {
+    // When they are not cached, the tables are planned here (segments, sizes,
+    // device buffers) and each segment's entries are computed and uploaded
+    // just before its launches, so the host fills segment i + 1 while the
+    // device runs segment i (the whole build used to precede the first
+    // launch: 21 ms on the host at 1M spins x 16 x 8000 samples).
+    SeqHost sq;
+    sq.nn = p->nn;
+    sq.gamma = p->gamma;
+    sq.dt = p->h_dt.data();
+    sq.gx = p->h_gx.data();
+    sq.gy = p->h_gy.data();
+    sq.gz = p->h_gz.data();
+    sq.coef = p->h_coef.data();
+    sq.adc = p->h_adc.data();
+    TableSet ts;
+    TableBuilder tb(sq, W.terms, W.split, W.NI, mflags, W.segs, ts);
+    const bool building = W.table_flags != mflags || W.table_parts != key;
+    if (building) {
         if (const int e = check_hip(p, hipStreamSynchronize(p->stream), "hipStreamSynchronize")) return e;
-        SeqHost sq;
-        sq.nn = p->nn;
-        sq.gamma = p->gamma;
-        sq.dt = p->h_dt.data();
-        sq.gx = p->h_gx.data();
-        sq.gy = p->h_gy.data();
-        sq.gz = p->h_gz.data();
-        sq.coef = p->h_coef.data();
-        sq.adc = p->h_adc.data();
-        TableSet ts;
-        build_tables(sq, W.terms, W.split, W.NI, mflags, pf, pl, rf, np, W.segs, ts);
-        rc = dev_upload(p, &W.kr, ts.kr);
-        rc = rc ? rc : dev_upload(p, &W.kref, ts.kref);
-        rc = rc ? rc : dev_upload(p, &W.bho, ts.bho);
-        rc = rc ? rc : dev_upload(p, &W.anc, ts.anc);
-        rc = rc ? rc : dev_upload(p, &W.tk, ts.tk);
-        rc = rc ? rc : dev_upload(p, &W.kr0, ts.kr0);
-        rc = rc ? rc : dev_upload(p, &W.kend, ts.kend);
+        W.table_flags = -1;  // valid again once every segment is uploaded
+        tb.plan(pf, pl, rf, np);
+        rc = dev_alloc(p, &W.kref, ts.kref.size());
+        rc = rc ? rc : dev_alloc(p, &W.bho, ts.bho.size());
+        rc = rc ? rc : dev_alloc(p, &W.anc, ts.anc.size());
+        rc = rc ? rc : dev_alloc(p, &W.tk, ts.tk.size());
+        rc = rc ? rc : dev_alloc(p, &W.kr0, ts.kr0.size());
+        rc = rc ? rc : dev_alloc(p, &W.kend, ts.kend.size());
+        free_dev(W.kr);  // allocated when the first segment falls back
         if (rc) return rc;
-        W.table_flags = mflags;
-        W.table_parts = key;
     }
     // Spin tiles (16-spin groups per work item) of the segments; partial and
     // seed buffers sized for the largest segment.
@@ -776,6 +810,10 @@ int mixed_exec(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t 
     }
 
     for (size_t si = 0; si < W.segs.size(); ++si) {
+        if (building && !W.segs[si].rf) {
+            tb.fill(si);
+            if ((rc = upload_segment_tables(p, W, ts, W.segs[si], tb.kr_bound()))) return rc;
+        }
         const Segment &s = W.segs[si];
         if (s.rf) {
             // RF blocks: f64 per-block excitation (+ reduction of RF ADC samples).
@@ -877,6 +915,10 @@ int mixed_exec(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t 
             rc = check_hip(p, hipGetLastError(), "mixed segment end");
             if (rc) return rc;
         }
+    }
+    if (building) {
+        W.table_flags = mflags;
+        W.table_parts = key;
     }
     return HOM_OK;
 }
