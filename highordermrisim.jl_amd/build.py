@@ -15,7 +15,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libhom.so"
-SOURCES = ["hom_abi.hip", "plain_kernels.hip", "mixed_kernels.hip", "serve.hip", "kspace.cpp", "discretize.cpp"]
+SOURCES = ["hom_abi.hip", "plain_kernels.hip", "mixed_kernels.hip", "serve.hip", "staging.cpp", "kspace.cpp", "discretize.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc" if Path("/opt/rocm/bin/hipcc").exists()
                        else "hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <type_traits>
 
 using namespace hom;
 
@@ -20,14 +21,15 @@ thread_local std::string g_create_error;
 template <typename T>
 int upload_vec(hom_plan *p, void **dst, const double *src, int64_t n) {
     if (n == 0) return HOM_OK;
-    std::vector<T> tmp((size_t)n);
-    for (int64_t i = 0; i < n; ++i) tmp[(size_t)i] = (T)src[i];
     if (!*dst) {
         hipError_t e = hom::dev_malloc(dst, sizeof(T) * (size_t)n);
         if (e != hipSuccess) return set_error(p, HOM_ERR_MEMORY, "hipMalloc failed for an input array");
     }
-    return check_hip(p, hipMemcpy(*dst, tmp.data(), sizeof(T) * (size_t)n, hipMemcpyHostToDevice),
-                     "hipMemcpy");
+    const size_t bytes = sizeof(T) * (size_t)n;
+    if (std::is_same<T, double>::value) return staged_upload(p, *dst, bytes, src, bytes, bytes, 1);
+    std::vector<T> tmp((size_t)n);
+    for (int64_t i = 0; i < n; ++i) tmp[(size_t)i] = (T)src[i];
+    return staged_upload(p, *dst, bytes, tmp.data(), bytes, bytes, 1);
 }
 
 int upload_real(hom_plan *p, void **dst, const double *src, int64_t n) {
@@ -334,6 +336,7 @@ int hom_plan_destroy(hom_plan *p) {
     (void)hipSetDevice(p->device);
     if (p->stream) (void)hipStreamSynchronize(p->stream);  // releases proceed regardless
     mixed_free(p);
+    staging_release(p);
     void **all[] = {&p->d.x,  &p->d.y,  &p->d.z,  &p->d.rho, &p->d.T1, &p->d.T2,  &p->d.dw,
                     &p->d.csm, &p->d.coef, &p->d.gx, &p->d.gy, &p->d.gz, &p->d.dt, &p->d.b1,
                     &p->d.df, &p->d.psi, &p->d.mxy, &p->d.mz, &p->d.states, &p->d.partials,
@@ -402,12 +405,11 @@ static int hom_upload_csm_impl(hom_plan *p, const double *csm, int64_t ld) {
                     tmp[2 * (s + c * p->ns)] = (float)v[0];
                     tmp[2 * (s + c * p->ns) + 1] = (float)v[1];
                 }
-            rc = check_hip(p, hipMemcpy(p->d.csm, tmp.data(), 4 * 2 * n, hipMemcpyHostToDevice), "hipMemcpy");
+            rc = staged_upload(p, p->d.csm, 8 * n, tmp.data(), 8 * n, 8 * n, 1);
         } else {
             // straight from the caller's columns (column stride ld) into stride ns
-            rc = check_hip(p, hipMemcpy2D(p->d.csm, 16 * (size_t)p->ns, csm, 16 * (size_t)ld, 16 * (size_t)p->ns,
-                                          (size_t)p->nc, hipMemcpyHostToDevice),
-                           "hipMemcpy2D");
+            rc = staged_upload(p, p->d.csm, 16 * (size_t)p->ns, csm, 16 * (size_t)ld, 16 * (size_t)p->ns,
+                               (size_t)p->nc);
         }
         // column maxima of |csm| for the mixed path's power-of-two scales
         std::vector<unsigned long long> bits((size_t)p->nc, 0ull);
@@ -500,15 +502,15 @@ static int hom_get_state_impl(hom_plan *p, double *mxy, double *mz) {
     const size_t ns = (size_t)p->ns;
     if (p->is_f32()) {
         std::vector<float> a(2 * ns), b(ns);
-        rc = check_hip(p, hipMemcpy(a.data(), p->d.mxy, 8 * ns, hipMemcpyDeviceToHost), "hipMemcpy");
-        if (!rc) rc = check_hip(p, hipMemcpy(b.data(), p->d.mz, 4 * ns, hipMemcpyDeviceToHost), "hipMemcpy");
+        rc = staged_download(p, a.data(), 8 * ns, p->d.mxy, 8 * ns, 8 * ns, 1);
+        if (!rc) rc = staged_download(p, b.data(), 4 * ns, p->d.mz, 4 * ns, 4 * ns, 1);
         if (rc) return rc;
         for (size_t i = 0; i < 2 * ns; ++i) mxy[i] = a[i];
         for (size_t i = 0; i < ns; ++i) mz[i] = b[i];
         return HOM_OK;
     }
-    rc = check_hip(p, hipMemcpy(mxy, p->d.mxy, 16 * ns, hipMemcpyDeviceToHost), "hipMemcpy");
-    return rc ? rc : check_hip(p, hipMemcpy(mz, p->d.mz, 8 * ns, hipMemcpyDeviceToHost), "hipMemcpy");
+    rc = staged_download(p, mxy, 16 * ns, p->d.mxy, 16 * ns, 16 * ns, 1);
+    return rc ? rc : staged_download(p, mz, 8 * ns, p->d.mz, 8 * ns, 8 * ns, 1);
 }
 
 static int hom_precession_block_impl(hom_plan *p, int64_t first, int64_t last, int flags, void *out, int64_t ld) {
@@ -622,9 +624,8 @@ int hom_download_signal(hom_plan *p, void *host, int64_t ld) {
     int rc = enter_device(p, true);
     if (rc) return rc;
     const size_t es = p->complex_size();
-    return check_hip(p, hipMemcpy2D(host, es * (size_t)ld, p->signal, es * (size_t)p->signal_rows,
-                                    es * (size_t)p->signal_rows, (size_t)p->nc, hipMemcpyDeviceToHost),
-                     "hipMemcpy2D");
+    return staged_download(p, host, es * (size_t)ld, p->signal, es * (size_t)p->signal_rows,
+                           es * (size_t)p->signal_rows, (size_t)p->nc);
 }
 
 static int hom_buffer_alloc_impl(hom_plan *p, int64_t bytes, void **dev) {
@@ -656,14 +657,14 @@ int hom_buffer_upload(hom_plan *p, void *dev, const void *host, int64_t bytes) {
     if (!p || bytes < 0 || (bytes && (!dev || !host))) return HOM_ERR_ARGUMENT;
     if (!bytes) return HOM_OK;
     if (const int e = enter_device(p, true)) return e;
-    return check_hip(p, hipMemcpy(dev, host, (size_t)bytes, hipMemcpyHostToDevice), "hipMemcpy");
+    return staged_upload(p, dev, (size_t)bytes, host, (size_t)bytes, (size_t)bytes, 1);
 }
 
 int hom_buffer_download(hom_plan *p, void *host, const void *dev, int64_t bytes) {
     if (!p || bytes < 0 || (bytes && (!dev || !host))) return HOM_ERR_ARGUMENT;
     if (!bytes) return HOM_OK;
     int rc = enter_device(p, true);
-    return rc ? rc : check_hip(p, hipMemcpy(host, dev, (size_t)bytes, hipMemcpyDeviceToHost), "hipMemcpy");
+    return rc ? rc : staged_download(p, host, (size_t)bytes, dev, (size_t)bytes, (size_t)bytes, 1);
 }
 
 static int hom_plan_set_timing_impl(hom_plan *p, int enable) {

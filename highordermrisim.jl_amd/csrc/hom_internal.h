@@ -81,6 +81,13 @@ struct TimerSlot {
     int64_t folded_launches = 0;
 };
 
+// Two pinned chunk buffers and their copy-done events for staged transfers
+// (allocated on the first large copy, released with the plan).
+struct Staging {
+    void *buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
 }  // namespace hom
 
 struct hom_plan {
@@ -119,6 +126,7 @@ struct hom_plan {
     hom::TimerSlot timers[HOM_K_COUNT];
 
     std::vector<void *> user_buffers;
+    hom::Staging staging;
 
     bool is_f32() const { return precision == HOM_F32; }
     // f32/f64 plan whose precession blocks run the fused kernel
@@ -162,6 +170,19 @@ int ensure_state_cols(hom_plan *p, int64_t n_adc);
 // Makes the plan's device current and, with sync, waits for its stream (so
 // that an earlier asynchronous failure is reported here); returns a status.
 int enter_device(hom_plan *p, bool sync);
+
+// ---- staged host<->device copies (staging.cpp) ----
+// Copies rows x width bytes between pageable host memory (row stride
+// host_pitch) and device memory (row stride dev_pitch) on the plan's stream
+// and returns when the copy has finished.  Copies of 4 MiB and more go
+// through the plan's pinned chunk buffers, packed or unpacked by host
+// threads while the previous chunk is in flight; smaller ones (or all of
+// them, when pinned memory cannot be had) go through hipMemcpy2DAsync.
+int staged_upload(hom_plan *p, void *dev, size_t dev_pitch, const void *host, size_t host_pitch, size_t width,
+                  size_t rows);
+int staged_download(hom_plan *p, void *host, size_t host_pitch, const void *dev, size_t dev_pitch, size_t width,
+                    size_t rows);
+void staging_release(hom_plan *p);
 
 // ---- plain kernels (plain_kernels.hip) ----
 // Block [first, last] of free precession / excitation for all spins, writing
