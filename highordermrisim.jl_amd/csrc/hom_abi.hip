@@ -170,8 +170,9 @@ int enter_device(hom_plan *p, bool sync) {
     return rc;
 }
 
-ScopedTimer::ScopedTimer(hom_plan *plan, int kernel_id) : p(plan), id(kernel_id) {
+ScopedTimer::ScopedTimer(hom_plan *plan, int kernel_id, hipStream_t stream) : p(plan), id(kernel_id) {
     if (!p || !p->timing) return;
+    st = stream ? stream : p->stream;
     TimerSlot &s = p->timers[id];
     if (s.used == s.starts.size()) {
         hipEvent_t a = nullptr, b = nullptr;
@@ -179,13 +180,13 @@ ScopedTimer::ScopedTimer(hom_plan *plan, int kernel_id) : p(plan), id(kernel_id)
         s.starts.push_back(a);
         s.stops.push_back(b);
     }
-    (void)hipEventRecord(s.starts[s.used], p->stream);  // timing only
+    (void)hipEventRecord(s.starts[s.used], st);  // timing only
     stop_ev = s.stops[s.used];
 }
 
 ScopedTimer::~ScopedTimer() {
     if (!stop_ev) return;
-    (void)hipEventRecord(stop_ev, p->stream);  // timing only
+    (void)hipEventRecord(stop_ev, st);  // timing only
     p->timers[id].used++;
 }
 

@@ -139,12 +139,14 @@ struct hom_plan {
 
 namespace hom {
 
-// Records a start/stop event pair around one launch when timing is on.
+// Records a start/stop event pair around one launch when timing is on, on
+// the given stream (default: the plan's).
 struct ScopedTimer {
     hom_plan *p;
     int id;
+    hipStream_t st = nullptr;
     hipEvent_t stop_ev = nullptr;
-    ScopedTimer(hom_plan *plan, int kernel_id);
+    ScopedTimer(hom_plan *plan, int kernel_id, hipStream_t stream = nullptr);
     ~ScopedTimer();
 };
 

@@ -47,11 +47,14 @@ struct MixedWorkspace {
     float *cdec = nullptr;      // ngroups x 16, MFMA output-row order
     half8_t *bop = nullptr;     // ngroups x NCB x 2 x 64
     double *csc = nullptr;      // 8 * NCB power-of-two column scales
-    double *th0 = nullptr;      // ngroups x 16: arg(M0) / 2pi
-    float *lm = nullptr;        // ngroups x 16: log2(|M0| / S) + kOpScale
-    float *m0max = nullptr;     // max |M0| of the current segment
-    double *seed = nullptr;     // max_nab x ngroups x 16 + 16 padding
-    int64_t seed_cap = 0;
+    // Per-segment operands in two slots (segment k with samples uses slot
+    // k % 2, so that the small kernels of one segment run beside the main
+    // kernel of the previous one):
+    double *th0 = nullptr;      // 2 x ngroups x 16: arg(M0) / 2pi
+    float *lm = nullptr;        // 2 x ngroups x 16: log2(|M0| / S) + kOpScale
+    float *m0max = nullptr;     // 2: max |M0| of the segment
+    double *seed = nullptr;     // 2 x (max_nab x ngroups x 16 + 16 padding)
+    int64_t seed_cap = 0;       // per slot
     double *kr = nullptr;       // fallback FP64 tables (all terms)
     double *kref = nullptr;     // FP64 Kr of (x, y, z, dw) at the tile references
     half8_t *bho = nullptr;     // FP16 offset tables
@@ -59,9 +62,13 @@ struct MixedWorkspace {
     float *tk = nullptr;        // per-sample time within segment
     double *kr0 = nullptr;      // per-sample zeroth-order phase (revolutions)
     double *kend = nullptr;     // per segment: all terms, then kr0_end
-    float *partials = nullptr;
-    int64_t partial_cap = 0;
+    float *partials = nullptr;  // 2 slots
+    int64_t partial_cap = 0;    // per slot
     std::vector<Segment> segs;
+    // The main kernels' stream, and the events that order it against the
+    // plan's stream (prep done / main done per slot; the end-of-run join).
+    hipStream_t main_stream = nullptr;
+    hipEvent_t prep_done[2] = {nullptr, nullptr}, main_done[2] = {nullptr, nullptr}, joined = nullptr;
 };
 
 TermList make_terms(int flags) {
@@ -619,6 +626,12 @@ void mixed_free(hom_plan *p) {
     free_dev(W->bop); free_dev(W->csc); free_dev(W->th0); free_dev(W->lm); free_dev(W->m0max);
     free_dev(W->seed); free_dev(W->kr); free_dev(W->kref); free_dev(W->bho); free_dev(W->anc);
     free_dev(W->tk); free_dev(W->kr0); free_dev(W->kend); free_dev(W->partials);
+    for (int b = 0; b < 2; ++b) {
+        if (W->prep_done[b]) (void)hipEventDestroy(W->prep_done[b]);
+        if (W->main_done[b]) (void)hipEventDestroy(W->main_done[b]);
+    }
+    if (W->joined) (void)hipEventDestroy(W->joined);
+    if (W->main_stream) (void)hipStreamDestroy(W->main_stream);
     delete W;
     p->mixed = nullptr;
 }
@@ -668,6 +681,20 @@ int segment_spin_tile(const hom_plan *p, int64_t ngroups, int64_t nsb) {
     return best;
 }
 
+// On destruction, when the second stream was used: the first stream waits
+// for everything queued so far on the second (or, if that cannot be
+// queued, the host waits for the second stream).
+struct StreamJoin {
+    hipStream_t first, second;
+    hipEvent_t ev;
+    bool used = false;
+    ~StreamJoin() {
+        if (!used) return;
+        if (hipEventRecord(ev, second) != hipSuccess || hipStreamWaitEvent(first, ev, 0) != hipSuccess)
+            (void)hipStreamSynchronize(second);
+    }
+};
+
 // The block loop of the mixed path over parts [pf, pl] (rf marks excitation
 // parts), writing sum(ADC) x Nc complex double rows into sig (column stride ld).
 int mixed_exec(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t *rf, int64_t np,
@@ -700,9 +727,9 @@ int mixed_exec(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t 
             hom::dev_malloc((void **)&W.cdec, sizeof(float) * (size_t)(ngroups * 16)) != hipSuccess ||
             hom::dev_malloc((void **)&W.bop, sizeof(half8_t) * (size_t)(ngroups * NCB * 2 * 64)) != hipSuccess ||
             hom::dev_malloc((void **)&W.csc, sizeof(double) * (size_t)(8 * NCB)) != hipSuccess ||
-            hom::dev_malloc((void **)&W.th0, sizeof(double) * (size_t)(ngroups * 16)) != hipSuccess ||
-            hom::dev_malloc((void **)&W.lm, sizeof(float) * (size_t)(ngroups * 16)) != hipSuccess ||
-            hom::dev_malloc((void **)&W.m0max, sizeof(float)) != hipSuccess) {
+            hom::dev_malloc((void **)&W.th0, sizeof(double) * (size_t)(2 * ngroups * 16)) != hipSuccess ||
+            hom::dev_malloc((void **)&W.lm, sizeof(float) * (size_t)(2 * ngroups * 16)) != hipSuccess ||
+            hom::dev_malloc((void **)&W.m0max, sizeof(float) * 2) != hipSuccess) {
             W.basis_flags = -1;
             return set_error(p, HOM_ERR_MEMORY, "device allocation failed for mixed-path operands");
         }
@@ -792,7 +819,7 @@ int mixed_exec(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t 
         if (need > W.partial_cap) {
             free_dev(W.partials);
             W.partial_cap = 0;
-            if (hom::dev_malloc((void **)&W.partials, sizeof(float) * (size_t)need) != hipSuccess) {
+            if (hom::dev_malloc((void **)&W.partials, sizeof(float) * (size_t)(2 * need)) != hipSuccess) {
                 W.partials = nullptr;
                 return set_error(p, HOM_ERR_MEMORY, "device allocation failed for mixed partials");
             }
@@ -801,7 +828,7 @@ int mixed_exec(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t 
         if (need_seed > W.seed_cap) {
             free_dev(W.seed);
             W.seed_cap = 0;
-            if (hom::dev_malloc((void **)&W.seed, sizeof(double) * (size_t)need_seed) != hipSuccess) {
+            if (hom::dev_malloc((void **)&W.seed, sizeof(double) * (size_t)(2 * need_seed)) != hipSuccess) {
                 W.seed = nullptr;
                 return set_error(p, HOM_ERR_MEMORY, "device allocation failed for phase seeds");
             }
@@ -809,6 +836,59 @@ int mixed_exec(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t 
         }
     }
 
+    // Two streams.  The main kernels run on W.main_stream; the RF blocks and
+    // each segment's prep, finalize and end-state kernels run on the plan's
+    // stream, so they fill the device beside a main kernel instead of
+    // between two.  Segment k with samples uses slot k % 2 of th0, lm,
+    // m0max, seed and partials.  The plan's stream runs prep(k),
+    // finalize(k - 1) (after main(k - 1)), end(k), prep(k + 1), ...: prep(k + 1)
+    // overlaps main(k); slot k % 2 is rewritten by prep(k + 2) only after
+    // finalize(k), which waited for main(k); main(k + 2) waits for prep(k + 2)
+    // and so for finalize(k), the last reader of its partials.  Every exit
+    // makes the plan's stream wait for the main stream, so synchronising the
+    // plan's stream covers both.
+    if (!W.main_stream) {
+        hipStream_t st = nullptr;
+        hipEvent_t ev[5] = {};
+        bool ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
+        for (int i = 0; ok && i < 5; ++i) ok = hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) == hipSuccess;
+        if (!ok) {
+            for (hipEvent_t e : ev)
+                if (e) (void)hipEventDestroy(e);
+            if (st) (void)hipStreamDestroy(st);
+            (void)hipGetLastError();
+            return set_error(p, HOM_ERR_DEVICE, "stream or event creation failed for the mixed path");
+        }
+        W.main_stream = st;
+        W.prep_done[0] = ev[0];
+        W.prep_done[1] = ev[1];
+        W.main_done[0] = ev[2];
+        W.main_done[1] = ev[3];
+        W.joined = ev[4];
+    }
+    StreamJoin join{p->stream, W.main_stream, W.joined};
+    struct Fin {
+        bool on;
+        int slot;
+        int64_t nadc, nstiles, rows, s_off, row0;
+    };
+    Fin fin{};
+    int64_t nsamp_segs = 0;
+    auto finalize = [&](Fin &f) -> int {
+        if (!f.on) return HOM_OK;
+        f.on = false;
+        int r = check_hip(p, hipStreamWaitEvent(p->stream, W.main_done[f.slot], 0), "hipStreamWaitEvent");
+        if (r) return r;
+        {
+            ScopedTimer timer(p, HOM_K_MIXED_FINALIZE);
+            hipLaunchKernelGGL(k_mix_finalize, dim3(blocks_for(f.nadc * nc, 256)), dim3(256), 0, p->stream,
+                               f.nadc, nc, f.nstiles, f.rows, (int64_t)(16 * NCB),
+                               (const float *)(W.partials + f.slot * W.partial_cap),
+                               (const float *)(W.m0max + f.slot), (const double *)W.csc,
+                               (const double *)(W.kr0 + f.s_off), sig + f.row0, ld);
+        }
+        return check_hip(p, hipGetLastError(), "mixed finalize");
+    };
     for (size_t si = 0; si < W.segs.size(); ++si) {
         if (building && !W.segs[si].rf) {
             tb.fill(si);
@@ -842,23 +922,33 @@ int mixed_exec(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t 
         if (s.nadc > 0 && ns > 0) {
             ++p->mixed_segments;
             if (s.fallback) ++p->mixed_fp64_segments;
+            const int slot = (int)(nsamp_segs++ & 1);
+            double *th0 = W.th0 + slot * ngroups * 16;
+            float *lm = W.lm + slot * ngroups * 16;
+            float *m0max = W.m0max + slot;
+            double *seed = W.seed + slot * W.seed_cap;
+            float *partials = W.partials + slot * W.partial_cap;
             {
                 ScopedTimer timer(p, HOM_K_MIXED_PREP);
-                hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, p->stream, W.m0max, (int64_t)1, 0.0f);
+                hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, p->stream, m0max, (int64_t)1, 0.0f);
                 hipLaunchKernelGGL(k_mix_m0max, dim3((unsigned)std::min<int64_t>(blocks_for(ns, 256), 256)),
                                    dim3(256), 0, p->stream, ns,
-                                   (const double2 *)p->d.mxy, W.m0max);
+                                   (const double2 *)p->d.mxy, m0max);
                 hipLaunchKernelGGL(k_mix_seg, dim3(blocks_for(ngroups * 16, 256)), dim3(256), 0,
                                    p->stream, ns, ngroups, (const double2 *)p->d.mxy,
-                                   (const float *)W.m0max, W.th0, W.lm);
+                                   (const float *)m0max, th0, lm);
                 if (!s.fallback)
                     hipLaunchKernelGGL(k_mix_seed, dim3(blocks_for(ngroups * 16, 256)), dim3(256),
                                        0, p->stream, ns, ngroups, s.nab, W.split, dx, dy, dz, ddw,
-                                       (const double *)W.th0, (const double *)(W.anc + s.anc_off),
-                                       W.seed);
+                                       (const double *)th0, (const double *)(W.anc + s.anc_off),
+                                       seed);
             }
             rc = check_hip(p, hipGetLastError(), "mixed prep");
+            rc = rc ? rc : check_hip(p, hipEventRecord(W.prep_done[slot], p->stream), "hipEventRecord");
+            rc = rc ? rc : check_hip(p, hipStreamWaitEvent(W.main_stream, W.prep_done[slot], 0),
+                                     "hipStreamWaitEvent");
             if (rc) return rc;
+            join.used = true;
             const int gpt = seg_gpt[si];
             const int64_t nstiles = stiles_of(gpt);
             const int64_t nwork = s.nsb * nstiles;
@@ -876,32 +966,28 @@ int mixed_exec(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t 
                 o.basis64 = W.basis64;
                 o.aho = W.aho;
                 o.cdec = W.cdec;
-                o.lm = W.lm;
-                o.seed = W.seed;
+                o.lm = lm;
+                o.seed = seed;
                 o.bop = W.bop;
                 o.kref = W.kref + s.kref_off;
                 o.bho = W.bho + s.bho_off;
                 o.tk = W.tk + s.s_off;
-                o.partials = W.partials;
+                o.partials = partials;
                 o.basisA = W.basisA;
-                o.th0 = W.th0;
+                o.th0 = th0;
                 o.kr = s.fallback ? W.kr + s.kr_off : nullptr;
-                ScopedTimer timer(p, HOM_K_MIXED_SEGMENT);
-                rc = check_hip(p, launch_main(ncb, s.fallback, W.split.nmh, W.NI, p->stream, o, s.nsb,
+                ScopedTimer timer(p, HOM_K_MIXED_SEGMENT, W.main_stream);
+                rc = check_hip(p, launch_main(ncb, s.fallback, W.split.nmh, W.NI, W.main_stream, o, s.nsb,
                                               nwork),
                                "mixed main kernel");
                 if (rc) return rc;
             }
-            {
-                ScopedTimer timer(p, HOM_K_MIXED_FINALIZE);
-                hipLaunchKernelGGL(k_mix_finalize, dim3(blocks_for(s.nadc * nc, 256)), dim3(256), 0,
-                                   p->stream, s.nadc, nc, nstiles, s.tiles * 16, (int64_t)(16 * NCB),
-                                   (const float *)W.partials, (const float *)W.m0max,
-                                   (const double *)W.csc, (const double *)(W.kr0 + s.s_off),
-                                   sig + s.row0, ld);
-            }
-            rc = check_hip(p, hipGetLastError(), "mixed finalize");
+            rc = check_hip(p, hipEventRecord(W.main_done[slot], W.main_stream), "hipEventRecord");
             if (rc) return rc;
+            // the previous segment's finalize; this one's waits for the next
+            // segment's prep (or the end of the loop)
+            if ((rc = finalize(fin))) return rc;
+            fin = Fin{true, slot, s.nadc, nstiles, s.tiles * 16, s.s_off, s.row0};
         } else if (s.nadc > 0) {
             rc = check_hip(p, launch_fill_complex(p, sig + s.row0, s.nadc, nc, ld), "fill");
             if (rc) return rc;
@@ -916,6 +1002,7 @@ int mixed_exec(hom_plan *p, const int64_t *pf, const int64_t *pl, const uint8_t 
             if (rc) return rc;
         }
     }
+    if ((rc = finalize(fin))) return rc;
     if (building) {
         W.table_flags = mflags;
         W.table_parts = key;
