@@ -12,7 +12,9 @@
  *     [first, last]; the ADC flag of the shared first node belongs to the
  *     previous block (reference src/simulation/SimulatorCore.jl:115-117).
  *
- * One plan is bound to one device and one HIP stream and owns every device
+ * One plan is bound to one device and one HIP stream (a mixed plan runs its
+ * main kernels on a second stream of its own, which every call joins back
+ * into the plan's stream before it returns) and owns every device
  * allocation it makes.  Calls on one plan must be serialized by the caller;
  * different plans may be used from different threads.  Shapes are validated
  * before any allocation or launch.  No C++ exception leaves the library: a
@@ -198,7 +200,8 @@ int hom_buffer_free(hom_plan *plan, void *dev);
 int hom_buffer_upload(hom_plan *plan, void *dev, const void *host, int64_t bytes);
 int hom_buffer_download(hom_plan *plan, void *host, const void *dev, int64_t bytes);
 
-/* Per-kernel timing with HIP events on the plan's stream (off by default). */
+/* Per-kernel timing with HIP events on the stream each kernel runs on (off by
+ * default). */
 int hom_plan_set_timing(hom_plan *plan, int enable);
 int hom_plan_kernel_time(hom_plan *plan, int kernel_id, double *total_ms, int64_t *launches);
 
