@@ -35,10 +35,13 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 METRIC = "spin-samples/s (3rd-order, 32 coils, NRMSE≤1e-5 vs f64) + p50/p99 latency"
 
-# MI355X published peaks used by the ceiling model.
-P_FP16_MFMA = 2.5e15      # dense FP16 MFMA flop/s
-P_FP32_VEC = 157.3e12     # FP32 vector flop/s
-P_TRANS = 9.8e12          # transcendental ops/s (quarter rate)
+# MI355X peaks used by the ceiling model: 256 CUs x 4 SIMDs at the 2.4 GHz
+# peak clock.
+P_FP16_MFMA = 2.5e15      # dense FP16 MFMA flop/s (published)
+P_FP32_VEC = 157.3e12     # FP32 vector flop/s (published: FMA = 2 flops, packed FP32 = 2 lanes)
+P_FP32_LANE = 39.3e12     # FP32 VALU lane operations/s without packed FP32: 16 lanes per cycle per SIMD
+P_TRANS = 19.7e12         # v_sin/v_cos/v_exp_f32 lane operations/s: 8 lanes per cycle per SIMD
+                          # (measured 1.65-1.76e13/s with 4 waves per SIMD below the peak clock)
 
 
 def load_package():
@@ -97,17 +100,23 @@ def measured_counters(spin_samples_per_launch: float, kernel_s: float, workload:
     return {"source": str(COUNTERS_PROFILE.relative_to(ROOT)),
             "mfma_f16_tflops": f16 / 1e12, "mfma_f16_frac_of_peak": f16 / 2.5e15,
             "mfma_busy_frac_of_simd_cycles": float(vals["mfma_busy_frac"]),
-            "transcendentals_per_s": trans, "transcendental_frac_of_peak": trans / 9.8e12,
+            "transcendentals_per_s": trans, "transcendental_frac_of_peak": trans / P_TRANS,
             "valu_insts_per_tile": float(vals["valu_insts_per_tile"]),
             "simd_cycles_per_tile": float(vals["simd_cycles_per_tile"]),
             "shader_clock_ghz_under_load": float(vals["shader_clock_ghz"])}
 
 
-def ceiling(nc, nt):
-    """Ceiling model: t_min per spin-sample = max(3*8*Nc / P_fp16_mfma,
-    (2*Nt + 20) / P_fp32 + 3 / P_trans); algorithmic flops F = 8 Nc + 2 Nt + 14."""
-    t_mfma = 3 * 8 * nc / P_FP16_MFMA
-    t_valu = (2 * nt + 20) / P_FP32_VEC + 3 / P_TRANS
+def ceiling(nc, nt, nmh):
+    """Lower bound on the fused kernel's time per spin-sample, with the matrix
+    pipe and the VALU perfectly overlapped: t_min = max(t_mfma, t_valu).
+    t_mfma: the FP16x3 complex coil GEMM (3 x 8 Nc flops) and the split-FP16
+    phase-offset contraction (nmh 16x16x32 MFMAs per 256 spin-samples = 64 nmh
+    flops) at the FP16 MFMA peak.  t_valu: 3 transcendentals (sin, cos, exp2)
+    and 8 FP32 lane operations (phase add, decay argument, two magnitude
+    products, FP16 hi/lo split of the two state components).  Returns
+    (t_min, algorithmic flops F = 8 Nc + 2 Nt + 14)."""
+    t_mfma = (3 * 8 * nc + 64 * nmh) / P_FP16_MFMA
+    t_valu = 3 / P_TRANS + 8 / P_FP32_LANE
     return max(t_mfma, t_valu), 8 * nc + 2 * nt + 14
 
 
@@ -204,7 +213,7 @@ def main():
 
     # dominant kernel: per launch spin-samples = ns * (samples of one segment)
     nt = method.n_terms
-    t_min, f_alg = ceiling(nc, nt)
+    t_min, f_alg = ceiling(nc, nt, nmh)
     ss_per_launch = spin_samples / max(seg_n / max(args.steps, 1), 1)
     kern_rate = ss_per_launch / (seg_ms / max(seg_n, 1) / 1e3) if seg_n else 0.0
     # achieved / peak against the published dense FP16 MFMA peak (the pipe of
@@ -225,8 +234,8 @@ def main():
         "ceiling_model": {
             "spin_samples_per_s": 1.0 / t_min,
             "frac": kern_rate * t_min,
-            "formula": "t_min=max(24*Nc/P_fp16mfma,(2*Nt+20)/P_fp32+3/P_trans); "
-                       "F_alg=8*Nc+2*Nt+14 per spin-sample",
+            "formula": "t_min=max((24*Nc+64*nmh)/P_fp16mfma, 3/P_trans+8/P_fp32lane) with "
+                       "P_trans=19.7e12/s, P_fp32lane=39.3e12/s; F_alg=8*Nc+2*Nt+14 per spin-sample",
         },
         "counters": measured_counters(ss_per_launch, seg_ms / max(seg_n, 1) / 1e3, cfg.name),
     }
