@@ -691,11 +691,23 @@ __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem
     // with scalar adds, not a readfirstlane per copy), and no lane mask for
     // the copies that every thread makes (1.3 % faster at 32 coils).
     const int wbase = __builtin_amdgcn_readfirstlane((tid & ~63) * 16);
-    auto stage = [&](int64_t grp, unsigned char *dst) {
-        if (L::kRecChunks >= 256 || u0) copy16_to_lds(s0.p + grp * s0.stride, dst + wbase);
-        if (L::kStage > 1 && (L::kRecChunks >= 512 || u1))
-            copy16_to_lds(s1.p + grp * s1.stride, dst + wbase + 256 * 16);
-        if (L::kStage > 2 && u2) copy16_to_lds(s2.p + grp * s2.stride, dst + wbase + 512 * 16);
+    // The loop below stages the groups g0, g0 + 1, g0 + 2, ... in that order,
+    // each once, so the sources are running pointers advanced by one record
+    // stride per call: a 64-bit add instead of a per-lane 64-bit multiply-add
+    // per copy, and 213 instead of 225 VGPRs.  Two waves per SIMD then leave
+    // 80 VGPRs, room for the 48-64-VGPR kernels of the plan's other stream
+    // to run beside this one instead of waiting for its workgroups to end
+    // (about 1 % per step at 1M spins x 32 coils).
+    const int64_t g0 = it.stile * o.gpt;
+    const int64_t g1 = std::min<int64_t>(g0 + o.gpt, o.ngroups);
+    const unsigned char *q0 = s0.p + g0 * s0.stride, *q1 = s1.p + g0 * s1.stride, *q2 = s2.p + g0 * s2.stride;
+    auto stage = [&](unsigned char *dst) {
+        if (L::kRecChunks >= 256 || u0) copy16_to_lds(q0, dst + wbase);
+        if (L::kStage > 1 && (L::kRecChunks >= 512 || u1)) copy16_to_lds(q1, dst + wbase + 256 * 16);
+        if (L::kStage > 2 && u2) copy16_to_lds(q2, dst + wbase + 512 * 16);
+        q0 += s0.stride;
+        q1 += s1.stride;
+        q2 += s2.stride;
     };
 
     // First-order anchors of one group from its FP64 basis (A: spin l%16,
@@ -806,9 +818,7 @@ __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem
         }
     };
 
-    const int64_t g0 = it.stile * o.gpt;
-    const int64_t g1 = std::min<int64_t>(g0 + o.gpt, o.ngroups);
-    stage(g0, sm.rec0);
+    stage(sm.rec0);  // group g0
     if (w == 0) {
         // anchors of the first group (later groups' come with the records)
         const double ba = o.basis64[g0 * 64 + lane];
@@ -820,12 +830,12 @@ __device__ __forceinline__ void main_split_body(const MainOps &o, const WorkItem
     // wave computing the anchors of group grp + 1 during group grp
     auto next_wave = [&](int64_t grp) { return grp + 1 < g1 && w == (int)((grp + 1 - g0) & (kWaves - 1)); };
     for (int64_t grp = g0; grp < g1; grp += 2) {
-        if (grp + 1 < g1) stage(grp + 1, sm.rec1);
+        if (grp + 1 < g1) stage(sm.rec1);  // group grp + 1
         compute(sm.rec0, sm.anc0, sm.anc1, next_wave(grp));
         wait_vmem();
         __syncthreads();
         if (grp + 1 >= g1) break;
-        if (grp + 2 < g1) stage(grp + 2, sm.rec0);
+        if (grp + 2 < g1) stage(sm.rec0);  // group grp + 2
         compute(sm.rec1, sm.anc1, sm.anc0, next_wave(grp + 1));
         wait_vmem();
         __syncthreads();
